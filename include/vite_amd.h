@@ -1,0 +1,182 @@
+/*
+ * vite_amd — C ABI of the MI355X (gfx950) Louvain community-detection path.
+ *
+ * Every entry point returns an int status (VT_OK = 0, negative on error), never
+ * throws, and leaves a message retrievable with vt_last_error().  Host pointers are
+ * plain C arrays; no C++ or framework types cross this boundary.  One context owns one
+ * HIP stream on one device; calls on a context are synchronous at return and must not
+ * overlap from several host threads.
+ *
+ * Reference correspondence (ECP-ExaGraph/vite, file:line):
+ *   vt_graph_upload        DistGraph/Graph CSR (graph.hpp:85-86, distgraph.hpp:77-107);
+ *                          Edge = {GraphElem tail; GraphWeight weight} (edge.hpp:72-80)
+ *   vt_louvain_phase       distLouvainMethod / ...WithColoring / ...VertexOrder, all nine
+ *                          overloads (louvain.hpp:80-137; louvain.cpp:7-1938)
+ *   vt_sweep               distExecuteLouvainIteration over all vertices from a given state
+ *                          (louvain.cpp:2281-2466), for replay parity
+ *   vt_modularity          distComputeModularity (louvain.cpp:2468-2532)
+ *   vt_rebuild             distbuildNextLevelGraph (rebuild.hpp:109-112; rebuild.cpp:480-504)
+ *   vt_color               distColoringMultiHashMinMax (coloring.hpp:86; coloring.cpp:52-118)
+ *   vt_gen_vite_rgg        generateInMemGraph at one rank (distgraph.hpp:116; distgraph.cpp:391-983)
+ *   vt_gen_rgg, vt_gen_ba  the workload generators (no reference counterpart: Vite's RGG uses
+ *                          another radius and an O(n^2) pair test; its BA needs NetworKit)
+ *   vt_run                 the phase loop of main.cpp:242-513
+ */
+#ifndef VITE_AMD_H
+#define VITE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VT_OK 0
+#define VT_ERR_ARG (-1)      /* invalid argument or malformed CSR            */
+#define VT_ERR_HIP (-2)      /* HIP runtime error                            */
+#define VT_ERR_BUDGET (-3)   /* allocation would exceed the context budget   */
+#define VT_ERR_LIMIT (-4)    /* size beyond what this build supports         */
+#define VT_ERR_STATE (-5)    /* call out of order (e.g. no partition yet)    */
+
+typedef struct vt_ctx vt_ctx;
+typedef struct vt_graph vt_graph;
+
+/* 16-byte edge record, layout-identical to the reference Edge (edge.hpp:72-80). */
+typedef struct {
+    int64_t tail;
+    double weight;
+} vt_edge;
+
+/* ---- context ---------------------------------------------------------------- */
+/* hbm_budget_bytes = 0 means no limit beyond the device. */
+int vt_ctx_create(int device, uint64_t hbm_budget_bytes, vt_ctx** out);
+int vt_ctx_destroy(vt_ctx* ctx);
+const char* vt_last_error(const vt_ctx* ctx);
+/* Bytes currently allocated through this context (graphs + working state). */
+uint64_t vt_ctx_bytes_in_use(const vt_ctx* ctx);
+/* Per-kernel timing with HIP events on the context stream (0 = off). */
+int vt_ctx_set_profiling(vt_ctx* ctx, int on);
+/* Average duration (ms) and launch count of kernel `name` since profiling was enabled. */
+int vt_ctx_kernel_stats(const vt_ctx* ctx, const char* name, double* avg_ms, int64_t* count,
+                        double* total_ms);
+int vt_ctx_reset_stats(vt_ctx* ctx);
+
+/* ---- graphs ------------------------------------------------------------------- */
+/* Copies a host CSR (offsets[nv+1], edges[ne]) to the device.  Rejects offsets that are
+ * not monotone from 0 to ne and tails outside [0, nv). */
+int vt_graph_upload(vt_ctx* ctx, int64_t nv, int64_t ne, const int64_t* offsets,
+                    const vt_edge* edges, vt_graph** out);
+int vt_graph_info(const vt_graph* g, int64_t* nv, int64_t* ne);
+/* Copies the device CSR back; offsets[nv+1], edges[ne] are caller-allocated. */
+int vt_graph_download(vt_ctx* ctx, const vt_graph* g, int64_t* offsets, vt_edge* edges);
+int vt_graph_free(vt_ctx* ctx, vt_graph* g);
+
+/* ---- Louvain phase -------------------------------------------------------------- */
+#define VT_MODE_PLAIN 0   /* distLouvainMethod                     (louvain.cpp:7-591)     */
+#define VT_MODE_COLOR 1   /* distLouvainMethodWithColoring         (louvain.cpp:593-1263)  */
+#define VT_MODE_VORDER 2  /* distLouvainMethodVertexOrder          (louvain.cpp:1270-1938) */
+
+/* Bit flags. */
+#define VT_FLAG_SEQUENTIAL_SUMS 1  /* 2m and modularity as left folds over vertex order,
+                                      as one OpenMP thread of the reference computes them
+                                      (slow; for exact whole-run replay on small graphs) */
+
+typedef struct {
+    double lower;          /* `lower` argument                                          */
+    double thresh;         /* `thresh` argument                                         */
+    int32_t et_type;       /* 0 none; 1 ET(local), 2 ET-prob(local), 3 ET(count check),
+                              4 ET-prob(count check) -- main.cpp:350-365                */
+    int32_t mode;          /* VT_MODE_*                                                 */
+    double et_delta;       /* ETDelta for types 2 and 4                                 */
+    int64_t num_color;     /* numColor as the reference receives it (numColors + 1)     */
+    const int64_t* colors; /* vertexColor (host, nv entries, -1 = uncolored) or NULL    */
+    int32_t flags;         /* VT_FLAG_*                                                 */
+    int32_t max_iters;     /* safety cap on iterations, 0 = none                        */
+} vt_phase_params;
+
+/* Runs one phase.  cvect_out (host, nv entries) may be NULL: the partition then stays on
+ * the device only, attached to `g` for vt_rebuild.  q_out receives the return value of
+ * the reference function (prevMod), iters_out its `iters`. */
+int vt_louvain_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out,
+                     int32_t* iters_out, double* q_out);
+
+/* Per-iteration record of the last vt_louvain_phase call on this context. */
+int vt_last_phase_trace(const vt_ctx* ctx, int32_t cap, double* q, int64_t* moves, int32_t* n);
+
+/* One local-move sweep (every vertex, Jacobi) from a host-given state: currComm, community
+ * sizes and degrees (localCinfo), vertex degrees and 1/2m.  Writes targetComm and
+ * clusterWeight; dsize/ddeg (may be NULL) receive localCupdate folded per community in
+ * vertex order. */
+int vt_sweep(vt_ctx* ctx, const vt_graph* g, const int64_t* curr, const int64_t* csize,
+             const double* cdeg, const double* vdeg, double constant, int64_t* target_out,
+             double* cw_out, int64_t* dsize_out, double* ddeg_out);
+
+/* distComputeModularity on host arrays (fixed-order device tree, or left fold with
+ * VT_FLAG_SEQUENTIAL_SUMS). */
+int vt_modularity(vt_ctx* ctx, int64_t nv, const double* cw, const double* cdeg, double constant,
+                  int32_t flags, double* q_out);
+
+/* Vertex degrees (row left folds) and 1/2m. */
+int vt_init_degrees(vt_ctx* ctx, const vt_graph* g, int32_t flags, double* vdeg_out,
+                    double* constant_out);
+
+/* ---- coarsening ----------------------------------------------------------------- */
+/* Builds the next-level graph from a partition: cvect is a host array (nv entries) or NULL
+ * for the partition the last vt_louvain_phase left on `g`. */
+int vt_rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* cvect, vt_graph** out);
+
+/* ---- coloring ----------------------------------------------------------------------- */
+int vt_color(vt_ctx* ctx, const vt_graph* g, int64_t nhash, int32_t target_pct,
+             int32_t single_iter, int64_t* colors_out, int64_t* next_color_out);
+
+/* ---- generators (device) --------------------------------------------------------- */
+/* Seeded random geometric graph in [0,1]^2 with radius sqrt(2 ln n/(pi n)) and extra_pct
+ * percent extra uniform-random edges of weight U(0,1). */
+int vt_gen_rgg(vt_ctx* ctx, uint64_t seed, int64_t n, double extra_pct, vt_graph** out);
+/* Seeded Barabasi-Albert graph, m attachments per vertex, unit weights. */
+int vt_gen_ba(vt_ctx* ctx, uint64_t seed, int64_t n, int64_t m, vt_graph** out);
+
+/* Vite's own in-memory RGG at one rank (generateInMemGraph, distgraph.cpp:391-983): Lehmer
+ * LCG x_{k+1} = 16807 x_k mod (2^31 - 1) from x0 (utils.hpp:135-330), X = first n values,
+ * Y = next n, radius (sqrt(ln n/(3.14159 n)) + sqrt(2.0736/n))/2, rows sorted by tail.
+ * The reference draws x0 as the first word of std::seed_seq{1}. */
+int vt_gen_vite_rgg(vt_ctx* ctx, int64_t n, uint32_t x0, vt_graph** out);
+
+/* ---- whole run (main.cpp:242-513) -------------------------------------------------- */
+typedef struct {
+    double threshold;           /* -h (default 1e-6)          */
+    int32_t threshold_cycling;  /* -i                          */
+    int32_t et_type;            /* -t                          */
+    double et_delta;            /* -a (default 1.0)            */
+    int32_t coloring;           /* -c                          */
+    int32_t vertex_order;       /* -d                          */
+    int32_t max_colors;         /* -c/-d first argument (8)    */
+    int32_t single_color_iter;  /* -c/-d second argument       */
+    int32_t one_phase;          /* -p                          */
+    int32_t flags;              /* VT_FLAG_*                   */
+} vt_run_params;
+
+#define VT_MAX_LEVELS 256
+typedef struct {
+    int32_t nlevels;
+    double level_q[VT_MAX_LEVELS];
+    int32_t level_iters[VT_MAX_LEVELS];
+    int64_t level_nv[VT_MAX_LEVELS];
+    int64_t level_ne[VT_MAX_LEVELS];
+    int32_t ncalls;
+    double call_q[VT_MAX_LEVELS + 1];
+    int32_t call_iters[VT_MAX_LEVELS + 1];
+    double final_q;
+    int32_t total_iters;
+    int64_t num_colors;
+    double clustering_seconds;  /* wall time of the Louvain calls                  */
+    double total_seconds;       /* wall time of the whole loop incl. rebuilds      */
+} vt_run_result;
+
+/* Runs the phase loop on `g` (not modified). */
+int vt_run(vt_ctx* ctx, const vt_graph* g, const vt_run_params* p, vt_run_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -52,6 +52,7 @@ class _RunResult(C.Structure):
     _fields_ = [("nlevels", C.c_int), ("level_q", C.c_double * MAXL), ("level_iters", C.c_int * MAXL),
                 ("level_nv", C.c_int64 * MAXL), ("level_ne", C.c_int64 * MAXL), ("ncalls", C.c_int),
                 ("call_q", C.c_double * (MAXL + 1)), ("call_iters", C.c_int * (MAXL + 1)),
+                ("call_cvect_fnv", C.c_uint64 * (MAXL + 1)),
                 ("final_q", C.c_double), ("total_iters", C.c_int), ("num_colors", C.c_int64)]
 
 
@@ -268,6 +269,7 @@ class RunResult:
     final_q: float
     total_iters: int
     num_colors: int
+    call_cvect_fnv: list
 
 
 def run(g: Graph, threshold=1e-6, threshold_cycling=False, et_type=0, et_delta=1.0,
@@ -281,7 +283,7 @@ def run(g: Graph, threshold=1e-6, threshold_cycling=False, et_type=0, et_delta=1
     n, m = res.nlevels, res.ncalls
     return RunResult(list(res.level_q[:n]), list(res.level_iters[:n]), list(res.level_nv[:n]),
                      list(res.level_ne[:n]), list(res.call_q[:m]), list(res.call_iters[:m]),
-                     res.final_q, res.total_iters, res.num_colors)
+                     res.final_q, res.total_iters, res.num_colors, [int(x) for x in res.call_cvect_fnv[:m]])
 
 
 def fnv_i64(v) -> int:

@@ -617,7 +617,12 @@ int or_run(const or_graph* g0, const or_run_params* rp, or_run_result* res)
         res->level_nv[res->nlevels < OR_MAX_LEVELS ? res->nlevels : OR_MAX_LEVELS - 1] = dg.nv;
         res->level_ne[res->nlevels < OR_MAX_LEVELS ? res->nlevels : OR_MAX_LEVELS - 1] = dg.ne;
         currMod = or_louvain_phase(&dg, &pp, cvect, &iters, NULL);
-        if (res->ncalls <= OR_MAX_LEVELS) { res->call_q[res->ncalls] = currMod; res->call_iters[res->ncalls] = iters; res->ncalls++; }
+        if (res->ncalls <= OR_MAX_LEVELS) {
+            res->call_q[res->ncalls] = currMod;
+            res->call_iters[res->ncalls] = iters;
+            res->call_cvect_fnv[res->ncalls] = or_fnv_i64(cvect, dg.nv);
+            res->ncalls++;
+        }
         free(colors);
         colors = NULL;
         tot_iters += iters;
@@ -635,7 +640,12 @@ int or_run(const or_graph* g0, const or_run_params* rp, or_run_result* res)
                 pp.lower = currMod;
                 pp.thresh = 1.0E-6;
                 currMod = or_louvain_phase(&dg, &pp, cvect, &iters, NULL);
-                if (res->ncalls <= OR_MAX_LEVELS) { res->call_q[res->ncalls] = currMod; res->call_iters[res->ncalls] = iters; res->ncalls++; }
+                if (res->ncalls <= OR_MAX_LEVELS) {
+                    res->call_q[res->ncalls] = currMod;
+                    res->call_iters[res->ncalls] = iters;
+                    res->call_cvect_fnv[res->ncalls] = or_fnv_i64(cvect, dg.nv);
+                    res->ncalls++;
+                }
             }
             break;
         }

@@ -106,6 +106,7 @@ typedef struct {
     int ncalls;                          /* Louvain calls made                          */
     double call_q[OR_MAX_LEVELS + 1];
     int call_iters[OR_MAX_LEVELS + 1];
+    uint64_t call_cvect_fnv[OR_MAX_LEVELS + 1]; /* FNV-1a of the partition each call returned */
     double final_q;                      /* currMod at loop exit (MODS / time)          */
     int total_iters;
     int64_t num_colors;                  /* coloring result when -c/-d                  */

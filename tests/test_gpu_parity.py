@@ -1,0 +1,177 @@
+"""GPU parity: every device entry point against the CPU oracle on the same inputs.
+
+Integer and index outputs (targets, communities, CSR structure, colors) must be
+bit-identical.  FP64 outputs computed in the same order as the oracle (row folds, the
+per-community counters, the delta folds, the coarse-edge folds) must be bit-identical too;
+sums the device forms as fixed-order trees (2m, modularity) are compared with
+VT_FLAG_SEQUENTIAL_SUMS for exactness and without it within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import vite_amd
+from oracle import pyoracle as po
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload(ctx, name):
+    g = graphs.graph(name)
+    return g, ctx.upload(vite_amd.CSR(g.offs, g.edges))
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+def test_upload_roundtrip(ctx, name):
+    g, dg = _upload(ctx, name)
+    back = dg.download()
+    assert np.array_equal(back.offs, g.offs)
+    assert np.array_equal(back.edges.view(np.uint8), g.edges.view(np.uint8))
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+def test_init_degrees(ctx, name):
+    g, dg = _upload(ctx, name)
+    vdeg_o, c_o = po.init_degrees(g)
+    vdeg, c_seq = dg.init_degrees(sequential=True)
+    assert np.array_equal(vdeg, vdeg_o)
+    assert c_seq == c_o
+    _, c_tree = dg.init_degrees(sequential=False)
+    assert abs(c_tree - c_o) <= 1e-12 * abs(c_o)
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+@pytest.mark.parametrize("seed", [None, 1, 2])
+def test_sweep_replay(ctx, name, seed):
+    """One Jacobi sweep from the same state: targets, clusterWeight and the per-community
+    delta folds are bit-exact."""
+    g, dg = _upload(ctx, name)
+    if seed is None:
+        vdeg, c = po.init_degrees(g)
+        nv = g.nv
+        curr, csize, cdeg = np.arange(nv), np.ones(nv, np.int64), vdeg.copy()
+    else:
+        curr, csize, cdeg, vdeg, c = graphs.random_state(g, seed)
+    t_o, cw_o, ds_o, dd_o = po.sweep(g, curr, csize, cdeg, vdeg, c)
+    t, cw, ds, dd = dg.sweep(curr, csize, cdeg, vdeg, c)
+    assert np.array_equal(t, t_o)
+    assert np.array_equal(cw.view(np.uint64), cw_o.view(np.uint64))
+    assert np.array_equal(ds, ds_o)
+    assert np.array_equal(dd.view(np.uint64), dd_o.view(np.uint64))
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+def test_modularity(ctx, name):
+    g = graphs.graph(name)
+    curr, csize, cdeg, vdeg, c = graphs.random_state(g, 4)
+    _, cw, _, _ = po.sweep(g, curr, csize, cdeg, vdeg, c)
+    q_o = po.modularity(cw, cdeg, c)
+    assert ctx.modularity(cw, cdeg, c, sequential=True) == q_o
+    assert abs(ctx.modularity(cw, cdeg, c) - q_o) <= 1e-12
+
+
+PHASES = [("plain", 0), ("plain", 1), ("plain", 2), ("plain", 3), ("plain", 4),
+          ("color", 0), ("color", 1), ("color", 2), ("color", 4), ("vorder", 0), ("vorder", 3), ("vorder", 2)]
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+@pytest.mark.parametrize("mode,et", PHASES)
+def test_phase_exact(ctx, name, mode, et):
+    """A whole phase with the reference's summation order: return value, iteration count and
+    partition identical to the oracle."""
+    g, dg = _upload(ctx, name)
+    kw = dict(lower=-1.0, thresh=1e-6, et_type=et, et_delta=0.25)
+    if mode != "plain":
+        nc, colors = po.color(g, 4)
+        kw.update(mode=mode, num_color=nc + 1, colors=colors)
+    q_o, cv_o, it_o = po.louvain_phase(g, **kw)
+    q, cv, it = dg.louvain_phase(sequential=True, **kw)
+    assert it == it_o
+    assert np.array_equal(cv, cv_o)
+    assert q == q_o
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+def test_phase_tree_sums(ctx, name):
+    """Default device summation (fixed-order trees for 2m and modularity): modularity within
+    1e-6 of the oracle's."""
+    g, dg = _upload(ctx, name)
+    q_o, cv_o, it_o = po.louvain_phase(g)
+    q, cv, it = dg.louvain_phase()
+    assert abs(q - q_o) <= 1e-6
+    q2, _, _ = dg.louvain_phase()
+    assert q2 == q  # deterministic run to run
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+def test_rebuild(ctx, name):
+    g, dg = _upload(ctx, name)
+    _, cv, _ = po.louvain_phase(g)
+    r_o = po.rebuild(g, cv)
+    r = dg.rebuild(cv).download()
+    assert np.array_equal(r.offs, r_o.offs)
+    assert np.array_equal(r.edges["tail"], r_o.edges["tail"])
+    assert np.array_equal(r.edges["weight"].view(np.uint64), r_o.edges["weight"].view(np.uint64))
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+def test_rebuild_from_device_partition(ctx, name):
+    g, dg = _upload(ctx, name)
+    _, cv, _ = dg.louvain_phase(sequential=True)
+    r = dg.rebuild(None).download()
+    r_o = po.rebuild(g, cv)
+    assert np.array_equal(r.offs, r_o.offs)
+    assert np.array_equal(r.edges.view(np.uint8), r_o.edges.view(np.uint8))
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
+@pytest.mark.parametrize("nhash,single", [(4, False), (2, False), (4, True), (1, False)])
+def test_color(ctx, name, nhash, single):
+    g, dg = _upload(ctx, name)
+    nc_o, col_o = po.color(g, nhash, 70, single)
+    nc, col = dg.color(nhash, 70, single)
+    assert nc == nc_o
+    assert np.array_equal(col, col_o)
+
+
+@pytest.mark.parametrize("seed,n,extra", [(1, 1, 0.0), (2, 5000, 0.0), (3, 6000, 1.0), (4, 20000, 1.0), (5, 3000, 10.0)])
+def test_gen_rgg(ctx, seed, n, extra):
+    g_o = po.gen_rgg(seed, n, extra)
+    g = ctx.gen_rgg(seed, n, extra).download()
+    assert np.array_equal(g.offs, g_o.offs)
+    assert np.array_equal(g.edges.view(np.uint8), g_o.edges.view(np.uint8))
+
+
+@pytest.mark.parametrize("seed,n,m", [(1, 1, 1), (2, 3000, 4), (3, 20000, 8)])
+def test_gen_ba(ctx, seed, n, m):
+    g_o = po.gen_ba(seed, n, m)
+    g = ctx.gen_ba(seed, n, m).download()
+    assert np.array_equal(g.offs, g_o.offs)
+    assert np.array_equal(g.edges.view(np.uint8), g_o.edges.view(np.uint8))
+
+
+RUNS = [dict(), dict(threshold_cycling=True), dict(coloring=True, et_type=2, et_delta=0.25),
+        dict(et_type=1), dict(vertex_order=True, et_type=3), dict(coloring=True, single_color_iter=True, et_type=4,
+                                                                   et_delta=0.5), dict(one_phase=True)]
+
+
+@pytest.mark.parametrize("name", ["rgg", "ba", "vite_rgg", "hub", "grid"])
+@pytest.mark.parametrize("kw", RUNS, ids=[str(i) for i in range(len(RUNS))])
+def test_run_exact(ctx, name, kw):
+    """The whole phase loop (main.cpp:242-513): every Louvain call's return value and
+    iteration count equal the oracle's when sums follow the reference order."""
+    g, dg = _upload(ctx, name)
+    r_o = po.run(g, **kw)
+    r = dg.run(sequential=True, **kw)
+    assert r.call_iters == r_o.call_iters
+    assert r.call_q == r_o.call_q
+    assert r.level_nv == r_o.level_nv
+    assert r.final_q == r_o.final_q
+
+
+@pytest.mark.parametrize("name", ["rgg", "ba", "vite_rgg"])
+def test_run_tree_sums(ctx, name):
+    g, dg = _upload(ctx, name)
+    r_o = po.run(g, threshold_cycling=True)
+    r = dg.run(threshold_cycling=True)
+    assert abs(r.final_q - r_o.final_q) <= 1e-6

@@ -1,0 +1,347 @@
+"""vite_amd: Louvain community detection on one MI355X (gfx950).
+
+Python view of the C ABI in include/vite_amd.h (ctypes over vite_amd/libvite_amd.so).
+The shared library is required: importing the package raises if it has not been built,
+there is no CPU fallback.
+
+Graphs use Vite's CSR layout (graph.hpp:85-86): offsets int64[nv+1] and 16-byte edge
+records {int64 tail, float64 weight} (edge.hpp:72-80).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvite_amd.so")
+
+EDGE_DTYPE = np.dtype([("tail", "<i8"), ("weight", "<f8")])
+
+VT_OK = 0
+VT_ERR_ARG = -1
+VT_ERR_HIP = -2
+VT_ERR_BUDGET = -3
+VT_ERR_LIMIT = -4
+VT_ERR_STATE = -5
+
+VT_MODE_PLAIN, VT_MODE_COLOR, VT_MODE_VORDER = 0, 1, 2
+VT_FLAG_SEQUENTIAL_SUMS = 1
+MAX_LEVELS = 256
+# First word std::seed_seq{1} generates: the LCG start of Vite's generator (utils.hpp:163-170).
+VITE_LCG_X0 = 1967017404
+
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+
+
+class VtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vite_amd error {code}: {msg}")
+        self.code = code
+
+
+class _PhaseParams(C.Structure):
+    _fields_ = [("lower", C.c_double), ("thresh", C.c_double), ("et_type", C.c_int32), ("mode", C.c_int32),
+                ("et_delta", C.c_double), ("num_color", C.c_int64), ("colors", _i64p), ("flags", C.c_int32),
+                ("max_iters", C.c_int32)]
+
+
+class _RunParams(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("threshold_cycling", C.c_int32), ("et_type", C.c_int32),
+                ("et_delta", C.c_double), ("coloring", C.c_int32), ("vertex_order", C.c_int32),
+                ("max_colors", C.c_int32), ("single_color_iter", C.c_int32), ("one_phase", C.c_int32),
+                ("flags", C.c_int32)]
+
+
+class _RunResult(C.Structure):
+    _fields_ = [("nlevels", C.c_int32), ("level_q", C.c_double * MAX_LEVELS),
+                ("level_iters", C.c_int32 * MAX_LEVELS), ("level_nv", C.c_int64 * MAX_LEVELS),
+                ("level_ne", C.c_int64 * MAX_LEVELS), ("ncalls", C.c_int32),
+                ("call_q", C.c_double * (MAX_LEVELS + 1)), ("call_iters", C.c_int32 * (MAX_LEVELS + 1)),
+                ("final_q", C.c_double), ("total_iters", C.c_int32), ("num_colors", C.c_int64),
+                ("clustering_seconds", C.c_double), ("total_seconds", C.c_double)]
+
+
+# Every symbol include/vite_amd.h declares, with its ctypes signature.
+_SIGS = {
+    "vt_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "vt_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "vt_last_error": (C.c_char_p, [C.c_void_p]),
+    "vt_ctx_bytes_in_use": (C.c_uint64, [C.c_void_p]),
+    "vt_ctx_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "vt_ctx_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, _f64p, _i64p, _f64p]),
+    "vt_ctx_reset_stats": (C.c_int, [C.c_void_p]),
+    "vt_graph_upload": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, _i64p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "vt_graph_info": (C.c_int, [C.c_void_p, _i64p, _i64p]),
+    "vt_graph_download": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.c_void_p]),
+    "vt_graph_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vt_louvain_phase": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_PhaseParams), _i64p,
+                                   C.POINTER(C.c_int32), _f64p]),
+    "vt_last_phase_trace": (C.c_int, [C.c_void_p, C.c_int32, _f64p, _i64p, C.POINTER(C.c_int32)]),
+    "vt_sweep": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, _i64p, _f64p, _f64p, C.c_double, _i64p, _f64p,
+                           _i64p, _f64p]),
+    "vt_modularity": (C.c_int, [C.c_void_p, C.c_int64, _f64p, _f64p, C.c_double, C.c_int32, _f64p]),
+    "vt_init_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _f64p, _f64p]),
+    "vt_rebuild": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.POINTER(C.c_void_p)]),
+    "vt_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, _i64p, _i64p]),
+    "vt_gen_rgg": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_double, C.POINTER(C.c_void_p)]),
+    "vt_gen_ba": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
+    "vt_gen_vite_rgg": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "vt_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_RunParams), C.POINTER(_RunResult)]),
+}
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise RuntimeError(f"vite_amd: native library not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = load_library()
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _p(a, t=_i64p):
+    return a.ctypes.data_as(t)
+
+
+@dataclass
+class CSR:
+    """Host CSR in Vite's layout."""
+    offs: np.ndarray
+    edges: np.ndarray
+
+    @property
+    def nv(self) -> int:
+        return int(self.offs.shape[0] - 1)
+
+    @property
+    def ne(self) -> int:
+        return int(self.edges.shape[0])
+
+    @staticmethod
+    def from_arrays(offs, tails, weights) -> "CSR":
+        e = np.zeros(len(tails), dtype=EDGE_DTYPE)
+        e["tail"] = tails
+        e["weight"] = weights
+        return CSR(np.ascontiguousarray(offs, dtype=np.int64), e)
+
+
+class Context:
+    """One device, one HIP stream, an optional HBM byte budget."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        rc = _lib.vt_ctx_create(device, hbm_budget, C.byref(h))
+        if rc != VT_OK:
+            raise VtError(rc, "vt_ctx_create failed (no GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.vt_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int):
+        if rc != VT_OK:
+            msg = _lib.vt_last_error(self.h)
+            raise VtError(rc, msg.decode() if msg else "")
+
+    @property
+    def bytes_in_use(self) -> int:
+        return int(_lib.vt_ctx_bytes_in_use(self.h))
+
+    def set_profiling(self, on: bool):
+        self.check(_lib.vt_ctx_set_profiling(self.h, int(on)))
+
+    def kernel_stats(self, name: str):
+        avg, cnt, tot = C.c_double(), C.c_int64(), C.c_double()
+        self.check(_lib.vt_ctx_kernel_stats(self.h, name.encode(), C.byref(avg), C.byref(cnt), C.byref(tot)))
+        return avg.value, cnt.value, tot.value
+
+    def reset_stats(self):
+        self.check(_lib.vt_ctx_reset_stats(self.h))
+
+    # ---- graphs ----
+    def upload(self, g: CSR) -> "DeviceGraph":
+        h = C.c_void_p()
+        edges = np.ascontiguousarray(g.edges)
+        self.check(_lib.vt_graph_upload(self.h, g.nv, g.ne, _p(np.ascontiguousarray(g.offs, np.int64)),
+                                        edges.ctypes.data_as(C.c_void_p), C.byref(h)))
+        return DeviceGraph(self, h)
+
+    def gen_rgg(self, seed: int, n: int, extra_pct: float = 0.0) -> "DeviceGraph":
+        h = C.c_void_p()
+        self.check(_lib.vt_gen_rgg(self.h, seed, n, extra_pct, C.byref(h)))
+        return DeviceGraph(self, h)
+
+    def gen_ba(self, seed: int, n: int, m: int) -> "DeviceGraph":
+        h = C.c_void_p()
+        self.check(_lib.vt_gen_ba(self.h, seed, n, m, C.byref(h)))
+        return DeviceGraph(self, h)
+
+    def gen_vite_rgg(self, n: int, x0: int = VITE_LCG_X0) -> "DeviceGraph":
+        """Vite's own in-memory RGG at one rank (generateInMemGraph)."""
+        h = C.c_void_p()
+        self.check(_lib.vt_gen_vite_rgg(self.h, n, x0, C.byref(h)))
+        return DeviceGraph(self, h)
+
+    def modularity(self, cw, cdeg, constant: float, sequential: bool = False) -> float:
+        cw = np.ascontiguousarray(cw, np.float64)
+        cdeg = np.ascontiguousarray(cdeg, np.float64)
+        q = C.c_double()
+        self.check(_lib.vt_modularity(self.h, len(cw), _p(cw, _f64p), _p(cdeg, _f64p), constant,
+                                      VT_FLAG_SEQUENTIAL_SUMS if sequential else 0, C.byref(q)))
+        return q.value
+
+
+class DeviceGraph:
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+        nv, ne = C.c_int64(), C.c_int64()
+        _lib.vt_graph_info(h, C.byref(nv), C.byref(ne))
+        self.nv, self.ne = nv.value, ne.value
+
+    def free(self):
+        if self.h and self.ctx.h:
+            self.ctx.check(_lib.vt_graph_free(self.ctx.h, self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def download(self) -> CSR:
+        offs = np.zeros(self.nv + 1, np.int64)
+        edges = np.zeros(self.ne, EDGE_DTYPE)
+        self.ctx.check(_lib.vt_graph_download(self.ctx.h, self.h, _p(offs), edges.ctypes.data_as(C.c_void_p)))
+        return CSR(offs, edges)
+
+    def init_degrees(self, sequential: bool = False):
+        vdeg = np.zeros(self.nv)
+        c = C.c_double()
+        self.ctx.check(_lib.vt_init_degrees(self.ctx.h, self.h, VT_FLAG_SEQUENTIAL_SUMS if sequential else 0,
+                                            _p(vdeg, _f64p), C.byref(c)))
+        return vdeg, c.value
+
+    def louvain_phase(self, lower=-1.0, thresh=1e-6, et_type=0, et_delta=1.0, mode="plain", num_color=0,
+                      colors=None, sequential=False, max_iters=0, want_cvect=True):
+        m = {"plain": VT_MODE_PLAIN, "color": VT_MODE_COLOR, "vorder": VT_MODE_VORDER}[mode]
+        p = _PhaseParams(lower, thresh, et_type, m, et_delta, num_color, None,
+                         VT_FLAG_SEQUENTIAL_SUMS if sequential else 0, max_iters)
+        if colors is not None:
+            colors = np.ascontiguousarray(colors, np.int64)
+            p.colors = _p(colors)
+        cvect = np.zeros(self.nv, np.int64) if want_cvect else None
+        it, q = C.c_int32(), C.c_double()
+        self.ctx.check(_lib.vt_louvain_phase(self.ctx.h, self.h, C.byref(p), _p(cvect) if want_cvect else None,
+                                             C.byref(it), C.byref(q)))
+        return q.value, cvect, it.value
+
+    def last_trace(self, cap: int = 10000):
+        q = np.zeros(cap)
+        mv = np.zeros(cap, np.int64)
+        n = C.c_int32()
+        self.ctx.check(_lib.vt_last_phase_trace(self.ctx.h, cap, _p(q, _f64p), _p(mv), C.byref(n)))
+        k = min(n.value, cap)
+        return q[:k], mv[:k]
+
+    def sweep(self, curr, csize, cdeg, vdeg, constant):
+        nv = self.nv
+        curr = np.ascontiguousarray(curr, np.int64)
+        csize = np.ascontiguousarray(csize, np.int64)
+        cdeg = np.ascontiguousarray(cdeg, np.float64)
+        vdeg = np.ascontiguousarray(vdeg, np.float64)
+        target = np.zeros(nv, np.int64)
+        cw = np.zeros(nv)
+        dsize = np.zeros(nv, np.int64)
+        ddeg = np.zeros(nv)
+        self.ctx.check(_lib.vt_sweep(self.ctx.h, self.h, _p(curr), _p(csize), _p(cdeg, _f64p), _p(vdeg, _f64p),
+                                     constant, _p(target), _p(cw, _f64p), _p(dsize), _p(ddeg, _f64p)))
+        return target, cw, dsize, ddeg
+
+    def rebuild(self, cvect=None) -> "DeviceGraph":
+        h = C.c_void_p()
+        cp = None
+        if cvect is not None:
+            cvect = np.ascontiguousarray(cvect, np.int64)
+            cp = _p(cvect)
+        self.ctx.check(_lib.vt_rebuild(self.ctx.h, self.h, cp, C.byref(h)))
+        return DeviceGraph(self.ctx, h)
+
+    def color(self, nhash: int, target_pct: int = 70, single: bool = False):
+        colors = np.zeros(self.nv, np.int64)
+        nc = C.c_int64()
+        self.ctx.check(_lib.vt_color(self.ctx.h, self.h, nhash, target_pct, int(single), _p(colors), C.byref(nc)))
+        return nc.value, colors
+
+    def run(self, threshold=1e-6, threshold_cycling=False, et_type=0, et_delta=1.0, coloring=False,
+            vertex_order=False, max_colors=8, single_color_iter=False, one_phase=False, sequential=False):
+        rp = _RunParams(threshold, int(threshold_cycling), et_type, et_delta, int(coloring), int(vertex_order),
+                        max_colors, int(single_color_iter), int(one_phase),
+                        VT_FLAG_SEQUENTIAL_SUMS if sequential else 0)
+        res = _RunResult()
+        self.ctx.check(_lib.vt_run(self.ctx.h, self.h, C.byref(rp), C.byref(res)))
+        n, m = res.nlevels, res.ncalls
+        return RunResult(list(res.level_q[:n]), list(res.level_iters[:n]), list(res.level_nv[:n]),
+                         list(res.level_ne[:n]), list(res.call_q[:m]), list(res.call_iters[:m]), res.final_q,
+                         res.total_iters, res.num_colors, res.clustering_seconds, res.total_seconds)
+
+
+@dataclass
+class RunResult:
+    level_q: list
+    level_iters: list
+    level_nv: list
+    level_ne: list
+    call_q: list
+    call_iters: list
+    final_q: float
+    total_iters: int
+    num_colors: int
+    clustering_seconds: float = 0.0
+    total_seconds: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+
+def read_bin(path: str) -> CSR:
+    """Vite binary graph file (distgraph.cpp:149-150,181,220)."""
+    with open(path, "rb") as f:
+        nv, ne = np.fromfile(f, dtype=np.int64, count=2)
+        offs = np.fromfile(f, dtype=np.int64, count=int(nv) + 1)
+        edges = np.fromfile(f, dtype=EDGE_DTYPE, count=int(ne))
+    return CSR(offs - offs[0], edges)
+
+
+def write_bin(path: str, g: CSR) -> None:
+    """Vite binary graph file (writer: distgraph.cpp:1007-1045)."""
+    with open(path, "wb") as f:
+        np.array([g.nv, g.ne], dtype=np.int64).tofile(f)
+        g.offs.astype(np.int64).tofile(f)
+        np.ascontiguousarray(g.edges).tofile(f)

@@ -1,0 +1,181 @@
+// Internal declarations shared by the vite_amd translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vite_amd.h"
+
+namespace vt {
+
+// Device edge record: identical layout to vt_edge / the reference Edge (edge.hpp:72-80),
+// 16-byte aligned so a row element is one dwordx4 load.
+struct __align__(16) DEdge {
+    int64_t tail;
+    double weight;
+};
+static_assert(sizeof(DEdge) == 16, "edge record must be 16 bytes");
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+struct KernelStat {
+    int64_t count = 0;
+    double total_ms = 0.0;
+};
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t e0, e1;
+};
+
+}  // namespace vt
+
+struct vt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    std::string err;
+    // reusable scratch buffers keyed by purpose
+    std::map<std::string, std::pair<void*, size_t>> scratch;
+    // kernel timing
+    bool profiling = false;
+    std::vector<vt::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, vt::KernelStat> stats;
+    // per-iteration trace of the last phase
+    std::vector<double> trace_q;
+    std::vector<int64_t> trace_moves;
+    // pinned host staging for small readbacks
+    void* pinned = nullptr;
+};
+
+struct vt_graph {
+    int64_t nv = 0, ne = 0;
+    int64_t* offs = nullptr;     // device, nv+1
+    vt::DEdge* edges = nullptr;  // device, ne
+    int64_t max_degree = 0;
+    int64_t* part = nullptr;     // device partition left by the last phase (nv), or null
+    size_t bytes = 0;
+};
+
+namespace vt {
+
+// ---- errors -------------------------------------------------------------------------
+struct Fail {
+    int code;
+};
+
+[[noreturn]] void fail(vt_ctx* ctx, int code, const char* fmt, ...);
+void hip_check(vt_ctx* ctx, hipError_t e, const char* what, const char* file, int line);
+#define VT_HIP(ctx, call) ::vt::hip_check((ctx), (call), #call, __FILE__, __LINE__)
+
+// Runs `body` converting vt::Fail into the status code; the message stays on ctx.
+template <class F>
+int guarded(vt_ctx* ctx, F&& body)
+{
+    try {
+        body();
+        return VT_OK;
+    } catch (const Fail& f) {
+        return f.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = std::string("internal error: ") + e.what();
+        return VT_ERR_HIP;
+    }
+}
+
+// ---- memory ---------------------------------------------------------------------------
+void* dalloc(vt_ctx* ctx, size_t bytes);
+void dfree(vt_ctx* ctx, void* p, size_t bytes);
+// Grow-only named scratch buffer.
+void* scratch(vt_ctx* ctx, const std::string& key, size_t bytes);
+void release_scratch(vt_ctx* ctx);
+
+template <class T>
+struct DBuf {
+    vt_ctx* ctx = nullptr;
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(vt_ctx* c, size_t count) : ctx(c), n(count) { p = (T*)dalloc(c, sizeof(T) * (count ? count : 1)); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        reset();
+        ctx = o.ctx; p = o.p; n = o.n;
+        o.p = nullptr; o.n = 0;
+        return *this;
+    }
+    void reset()
+    {
+        if (p) dfree(ctx, p, sizeof(T) * (n ? n : 1));
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { reset(); }
+};
+
+// ---- launches with optional event timing ---------------------------------------------
+struct TimedScope {
+    vt_ctx* ctx;
+    const char* name;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TimedScope(vt_ctx* c, const char* n);
+    ~TimedScope();
+};
+
+void flush_timings(vt_ctx* ctx);
+
+#define VT_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                                  \
+    do {                                                                                      \
+        ::vt::TimedScope ts_(ctx, name);                                                      \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, (ctx)->stream, __VA_ARGS__); \
+        VT_HIP(ctx, hipGetLastError());                                                       \
+    } while (0)
+
+void sync(vt_ctx* ctx);
+void h2d(vt_ctx* ctx, void* dst, const void* src, size_t bytes);
+void d2h(vt_ctx* ctx, void* dst, const void* src, size_t bytes);
+void d2d(vt_ctx* ctx, void* dst, const void* src, size_t bytes);
+void dzero(vt_ctx* ctx, void* dst, size_t bytes);
+
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// ---- device primitives (vt_prims.hip) -------------------------------------------------
+// Deterministic fixed-order sums: every block reduces a fixed contiguous chunk with a
+// fixed tree; block partials are reduced by one block in index order.
+void tree_sum2(vt_ctx* ctx, const double* a, const double* b_sq, int64_t n, double* d_out2);
+void seq_sum2(vt_ctx* ctx, const double* a, const double* b_sq, int64_t n, double* d_out2);
+void exclusive_scan_i64(vt_ctx* ctx, const int64_t* in, int64_t* out, int64_t n);
+void exclusive_scan_i32(vt_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
+// Stable radix sort of (key, value) pairs on key bits [0, bits).
+void sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int bits,
+                    uint32_t* keys_alt, uint32_t* vals_alt, bool* result_in_alt);
+void sort_pairs_u64(vt_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits,
+                    uint64_t* keys_alt, uint32_t* vals_alt, bool* result_in_alt);
+
+// ---- graph helpers ---------------------------------------------------------------------
+vt_graph* graph_new(vt_ctx* ctx, int64_t nv, int64_t ne);
+void graph_finalize(vt_ctx* ctx, vt_graph* g);  // computes max_degree
+void graph_delete(vt_ctx* ctx, vt_graph* g);
+
+int64_t bits_for(int64_t maxval);
+
+}  // namespace vt

@@ -1,0 +1,930 @@
+// Louvain phase on one MI355X: distLouvainMethod and its eight sibling overloads
+// (louvain.cpp:7-1938) as one device-resident loop.
+//
+// Hot kernel: the local move (distExecuteLouvainIteration + distBuildLocalMapCounter +
+// distGetMaxIndex, louvain.cpp:2220-2466).  A group of G lanes owns one vertex at a time;
+// vertices are bucketed by degree so that G and the hash-table capacity fit the row.  The
+// per-vertex community table is open-addressed in LDS (HBM for hub rows).  Edges of a
+// row arrive G at a time (coalesced 16-byte records); every distinct community in a chunk
+// elects the lowest lane holding it, and that lane folds the chunk's weights for the
+// community in edge order onto the running value.  This reproduces the reference's left
+// fold in CSR order exactly, so the gains -- and therefore the moves -- are bit-identical.
+//
+// All FP64 here must be compiled with -ffp-contract=off: the reference build forms no
+// FMA, and a contracted gain can flip a near-tie.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vt_internal.hpp"
+
+namespace vt {
+
+// ------------------------------------------------------------------------------------------
+// Constants and small device helpers
+// ------------------------------------------------------------------------------------------
+
+// "No value yet" marker for a table slot: a quiet NaN with a payload no arithmetic makes.
+constexpr unsigned long long kSentBits = 0x7ff8dead5eed0001ull;
+
+__device__ __forceinline__ double sent_value() { return __longlong_as_double((long long)kSentBits); }
+__device__ __forceinline__ bool is_sent(double x) { return (unsigned long long)__double_as_longlong(x) == kSentBits; }
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t slot_hash(int32_t c) { return (uint32_t)c * 0x9E3779B1u; }
+
+// Degree buckets: (lanes per vertex G, table capacity CAP, max degree).  CAP >= 2*(deg+1)
+// keeps the table at most half full (a row has at most deg+1 distinct communities).
+constexpr int kNumBuckets = 7;
+constexpr int64_t kBucketMaxDeg[kNumBuckets] = {7, 15, 31, 63, 127, 1023, INT64_MAX};
+constexpr int kHubBucket = 6;
+
+__host__ __device__ inline int degree_bucket(int64_t d)
+{
+    return d <= 7 ? 0 : d <= 15 ? 1 : d <= 31 ? 2 : d <= 63 ? 3 : d <= 127 ? 4 : d <= 1023 ? 5 : 6;
+}
+
+template <int G> struct MaskT { using type = uint32_t; };
+template <> struct MaskT<64> { using type = unsigned long long; };
+
+// Table storage.  LDS tables use plain accesses (one wave owns a group; LDS operations of a
+// wave execute in order).  HBM tables use agent-scope atomics so no lane reads a stale L1
+// line written by another lane's atomic.
+template <bool HBM>
+struct TMem {
+    template <class T> __device__ static T ld(T* p)
+    {
+        if constexpr (HBM) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else return *p;
+    }
+    template <class T> __device__ static void st(T* p, T v)
+    {
+        if constexpr (HBM) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else *p = v;
+    }
+    __device__ static double ldd(double* p)
+    {
+        if constexpr (HBM) {
+            unsigned long long b = __hip_atomic_load((unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return __longlong_as_double((long long)b);
+        } else return *p;
+    }
+    __device__ static void std_(double* p, double v)
+    {
+        if constexpr (HBM)
+            __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else *p = v;
+    }
+};
+
+// Per-group table view.
+template <int G, bool HBM>
+struct Table {
+    using mask_t = typename MaskT<G>::type;
+    int32_t* keys;   // CAP, -1 = empty
+    double* vals;    // CAP, sentinel = empty
+    mask_t* masks;   // CAP (LDS) or unused (HBM)
+    int32_t* occ;    // CAP: occupied slot list
+    double* wst;     // G staged weights (LDS)
+    int32_t* sst;    // G staged slots (LDS, HBM path)
+    int32_t* nocc;   // 1 (LDS)
+    int32_t* cslot;  // 1 (LDS)
+    uint32_t capmask;
+};
+
+struct MoveArgs {
+    const int64_t* offs;
+    const DEdge* edges;
+    const int64_t* curr;
+    const double* vdeg;
+    const double* cdeg;
+    const int64_t* csize;
+    const unsigned char* active;  // null when ET is off
+    double* frozen;               // null when ET is off
+    int64_t* target;
+    double* cw;
+    const double* constant;       // device scalar 1/2m
+    unsigned long long* inactive_count;
+};
+
+// Inserts or finds community c; returns its slot.
+template <int G, bool HBM>
+__device__ __forceinline__ int32_t table_slot(const Table<G, HBM>& t, int32_t c)
+{
+    uint32_t h = (slot_hash(c) >> 7) & t.capmask;
+    for (;;) {
+        const int32_t k = TMem<HBM>::ld(&t.keys[h]);
+        if (k == c) return (int32_t)h;
+        if (k == -1) {
+            const int32_t old = atomicCAS(&t.keys[h], -1, c);
+            if (old == -1) {
+                const int32_t pos = atomicAdd(t.nocc, 1);
+                TMem<HBM>::st(&t.occ[pos], (int32_t)h);
+                return (int32_t)h;
+            }
+            if (old == c) return (int32_t)h;
+        }
+        h = (h + 1) & t.capmask;
+    }
+}
+
+// One vertex, G lanes.  `lane` is the lane index inside the group, `gbase` the group's
+// first lane inside the wave.  Follows louvain.cpp:2281-2466 and 2220-2279.
+template <int G, bool HBM>
+__device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HBM>& t, int32_t v, int lane,
+                                            int gbase, unsigned* inactive_local)
+{
+    using mask_t = typename MaskT<G>::type;
+    const mask_t gmask = (G == 64) ? (mask_t)~0ull : (mask_t)((1ull << G) - 1);
+
+    if (a.active && !a.active[v]) {
+        // Frozen vertex: clusterWeight[i] = frozenClusterWeight[i] (louvain.cpp:101-104).
+        if (lane == 0) {
+            a.cw[v] = a.frozen[v];
+            atomicAdd(inactive_local, 1u);
+        }
+        return;
+    }
+    const int64_t e0 = a.offs[v], e1 = a.offs[v + 1];
+    const int64_t cc = a.curr[v];
+    if (e0 == e1) {
+        // Isolated vertex: target = cc, clusterWeight stays 0 (louvain.cpp:2323-2335).
+        if (lane == 0) {
+            a.target[v] = cc;
+            a.cw[v] = 0.0;
+            if (a.frozen) a.frozen[v] = 0.0;
+        }
+        return;
+    }
+
+    // The own community is the first map entry, counter 0.0 (louvain.cpp:2326-2327).
+    if (lane == 0) {
+        const int32_t s = table_slot<G, HBM>(t, (int32_t)cc);
+        TMem<HBM>::std_(&t.vals[s], 0.0);
+        *t.cslot = s;
+    }
+    wave_sync();
+
+    double selfLoop = 0.0;
+    for (int64_t base = e0; base < e1; base += G) {
+        const int64_t j = base + lane;
+        const bool valid = j < e1;
+        int32_t slot = -1;
+        double w = 0.0;
+        bool self = false;
+        if (valid) {
+            const DEdge ed = a.edges[j];
+            w = ed.weight;
+            self = (ed.tail == (int64_t)v);
+            slot = table_slot<G, HBM>(t, (int32_t)a.curr[ed.tail]);
+        }
+        t.wst[lane] = w;
+        mask_t m = 0;
+        if constexpr (!HBM) {
+            if (valid) atomicOr(&t.masks[slot], (mask_t)1 << lane);
+            wave_sync();
+            if (valid) m = t.masks[slot];
+            wave_sync();
+        } else {
+            t.sst[lane] = slot;
+            wave_sync();
+            if (valid)
+                for (int k = 0; k < G; k++) m |= (t.sst[k] == slot) ? ((mask_t)1 << k) : (mask_t)0;
+            wave_sync();
+        }
+        // The lowest lane holding a community folds its weights in edge order.
+        if (valid && (m & (((mask_t)1 << lane) - 1)) == 0) {
+            double acc = TMem<HBM>::ldd(&t.vals[slot]);
+            acc = is_sent(acc) ? w : acc + w;
+            mask_t rest = m & (m - 1);
+            while (rest) {
+                const int k = (G == 64) ? __builtin_ctzll((unsigned long long)rest) : __builtin_ctz((unsigned)rest);
+                rest &= rest - 1;
+                acc = acc + t.wst[k];
+            }
+            TMem<HBM>::std_(&t.vals[slot], acc);
+            if constexpr (!HBM) t.masks[slot] = 0;
+        }
+        // selfLoop += weight for tail == vertex, in edge order (louvain.cpp:2440-2441).
+        mask_t sm = (mask_t)((__ballot(self) >> gbase) & gmask);
+        while (sm) {
+            const int k = (G == 64) ? __builtin_ctzll((unsigned long long)sm) : __builtin_ctz((unsigned)sm);
+            sm &= sm - 1;
+            selfLoop = selfLoop + t.wst[k];
+        }
+        wave_sync();
+    }
+
+    // distGetMaxIndex: strict-improvement argmax over candidates != cc with the
+    // smallest-id tie-break; only gains > 0 can win (louvain.cpp:2246-2273).
+    const int32_t nocc = *t.nocc;
+    const int32_t cs = *t.cslot;
+    const double kv = a.vdeg[v];
+    const double ccDeg = a.cdeg[cc];
+    const int64_t ccSize = a.csize[cc];
+    const double constant = *a.constant;
+    const double ecc = TMem<HBM>::ldd(&t.vals[cs]);
+    const double eix = ecc - selfLoop;
+    const double ax = ccDeg - kv;
+    double bg = 0.0;
+    long long bc = LLONG_MAX;
+    for (int p = lane; p < nocc; p += G) {
+        const int32_t s = TMem<HBM>::ld(&t.occ[p]);
+        const int32_t c = TMem<HBM>::ld(&t.keys[s]);
+        if ((int64_t)c == cc) continue;
+        const double ay = a.cdeg[c];
+        const double eiy = TMem<HBM>::ldd(&t.vals[s]);
+        const double gain = 2.0 * (eiy - eix) - 2.0 * kv * (ay - ax) * constant;
+        if (gain > 0.0 && (gain > bg || (gain == bg && (long long)c < bc))) {
+            bg = gain;
+            bc = c;
+        }
+    }
+    for (int o = G / 2; o > 0; o >>= 1) {
+        const double og = __shfl_xor(bg, o, G);
+        const long long oc = __shfl_xor(bc, o, G);
+        if (og > bg || (og == bg && oc < bc)) {
+            bg = og;
+            bc = oc;
+        }
+    }
+    if (lane == 0) {
+        int64_t tgt = cc;
+        int64_t maxSize = ccSize;
+        if (bg > 0.0) {
+            tgt = bc;
+            maxSize = a.csize[bc];
+        }
+        // Singleton rule (louvain.cpp:2275-2276).
+        if (maxSize == 1 && ccSize == 1 && tgt > cc) tgt = cc;
+        a.target[v] = tgt;
+        a.cw[v] = ecc;
+        if (a.frozen) a.frozen[v] = ecc;
+    }
+    // Restore the touched slots for the next vertex.
+    for (int p = lane; p < nocc; p += G) {
+        const int32_t s = TMem<HBM>::ld(&t.occ[p]);
+        TMem<HBM>::st(&t.keys[s], (int32_t)-1);
+        TMem<HBM>::std_(&t.vals[s], sent_value());
+    }
+    wave_sync();
+    if (lane == 0) *t.nocc = 0;
+    wave_sync();
+}
+
+// LDS-table kernel: BLOCK threads, BLOCK/G groups, each with its own CAP-slot table.
+template <int G, int CAP, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_move_lds(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
+{
+    using mask_t = typename MaskT<G>::type;
+    constexpr int NG = BLOCK / G;
+    __shared__ int32_t s_keys[NG][CAP];
+    __shared__ double s_vals[NG][CAP];
+    __shared__ mask_t s_masks[NG][CAP];
+    __shared__ int32_t s_occ[NG][CAP];
+    __shared__ double s_wst[NG][G];
+    __shared__ int32_t s_nocc[NG], s_cslot[NG];
+    __shared__ unsigned s_inactive;
+
+    const int g = threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    const int gbase = (threadIdx.x % 64) / G * G;
+    for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
+        (&s_keys[0][0])[i] = -1;
+        (&s_vals[0][0])[i] = sent_value();
+        (&s_masks[0][0])[i] = 0;
+    }
+    if (threadIdx.x < NG) s_nocc[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_inactive = 0;
+    __syncthreads();
+
+    Table<G, false> t{s_keys[g], s_vals[g], s_masks[g], s_occ[g], s_wst[g], nullptr, &s_nocc[g], &s_cslot[g],
+                      (uint32_t)(CAP - 1)};
+    const int64_t stride = (int64_t)gridDim.x * NG;
+    for (int64_t idx = (int64_t)blockIdx.x * NG + g; idx < n; idx += stride)
+        move_vertex<G, false>(a, t, list[idx], lane, gbase, &s_inactive);
+
+    __syncthreads();
+    if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
+}
+
+// Hub kernel: one wave per vertex, table in HBM (capacity `cap` per wave slot).
+__global__ void __launch_bounds__(64) k_move_hbm(MoveArgs a, const int32_t* __restrict__ list, int64_t n,
+                                                 int32_t* __restrict__ hkeys, double* __restrict__ hvals,
+                                                 int32_t* __restrict__ hocc, uint32_t cap)
+{
+    __shared__ double s_wst[64];
+    __shared__ int32_t s_sst[64];
+    __shared__ int32_t s_nocc, s_cslot;
+    __shared__ unsigned s_inactive;
+    const int lane = threadIdx.x;
+    int32_t* keys = hkeys + (size_t)blockIdx.x * cap;
+    double* vals = hvals + (size_t)blockIdx.x * cap;
+    int32_t* occ = hocc + (size_t)blockIdx.x * cap;
+    if (lane == 0) { s_nocc = 0; s_inactive = 0; }
+    __syncthreads();
+    Table<64, true> t{keys, vals, nullptr, occ, s_wst, s_sst, &s_nocc, &s_cslot, cap - 1};
+    for (int64_t idx = blockIdx.x; idx < n; idx += gridDim.x) move_vertex<64, true>(a, t, list[idx], lane, 0, &s_inactive);
+    __syncthreads();
+    if (lane == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
+}
+
+__global__ void k_hbm_table_init(int32_t* keys, double* vals, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        keys[i] = -1;
+        vals[i] = sent_value();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Init, bucketing, deltas, modularity, rotation
+// ------------------------------------------------------------------------------------------
+
+// distSumVertexDegree (louvain.cpp:2161-2186) + distInitComm (2564-2581) + ET state.
+__global__ void k_init(const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, int64_t nv,
+                       double* __restrict__ vdeg, double* __restrict__ cdeg, int64_t* __restrict__ csize,
+                       int64_t* __restrict__ past, int64_t* __restrict__ curr, int64_t* __restrict__ target,
+                       unsigned char* __restrict__ active, double* __restrict__ frozen,
+                       double* __restrict__ p_curr, double* __restrict__ p_prev)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        double tw = 0.0;
+        for (int64_t k = offs[i]; k < offs[i + 1]; k++) tw = tw + edges[k].weight;
+        vdeg[i] = tw;
+        cdeg[i] = tw;
+        csize[i] = 1;
+        past[i] = i;
+        curr[i] = i;
+        target[i] = 0;
+        if (active) {
+            active[i] = 1;
+            frozen[i] = 0.0;
+            p_curr[i] = 0.0;
+            p_prev[i] = 1.0;
+        }
+    }
+}
+
+__global__ void k_constant(const double* __restrict__ sums, double* __restrict__ constant)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) constant[0] = 1.0 / sums[0];
+}
+
+// Sort keys for the sweep lists: (color bucket, degree bucket).
+__global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const int64_t* __restrict__ colors,
+                            int64_t num_color, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)degree_bucket(offs[i + 1] - offs[i]);
+        uint32_t c = 0;
+        if (colors) c = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
+        keys[i] = c * kNumBuckets + b;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_color_keys(int64_t nv, const int64_t* __restrict__ colors, int64_t num_color,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        keys[i] = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_key_histogram(const uint32_t* __restrict__ keys, int64_t n, int64_t* __restrict__ hist)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd((unsigned long long*)&hist[keys[i]], 1ull);
+}
+
+// Movers in processing order: a vertex that was processed (active, or ET off) and whose
+// target differs from its community contributes two delta events (louvain.cpp:2343-2359).
+__global__ void k_mover_flags(const int32_t* __restrict__ order, int64_t n, const int64_t* __restrict__ target,
+                              const int64_t* __restrict__ curr, const unsigned char* __restrict__ active,
+                              int32_t* __restrict__ flags)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = order ? order[p] : p;
+        const bool live = !active || active[v];
+        flags[p] = (live && target[v] != curr[v]) ? 1 : 0;
+    }
+}
+
+__global__ void k_scan_total(const int32_t* __restrict__ flags, const int32_t* __restrict__ pos, int64_t n,
+                             int64_t* __restrict__ total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) total[0] = n ? (int64_t)pos[n - 1] + flags[n - 1] : 0;
+}
+
+__global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, const int32_t* __restrict__ flags,
+                              const int32_t* __restrict__ pos, const int64_t* __restrict__ target,
+                              const int64_t* __restrict__ curr, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        if (!flags[p]) continue;
+        const int64_t v = order ? order[p] : p;
+        const int64_t q = 2 * (int64_t)pos[p];
+        keys[q] = (uint32_t)target[v];
+        vals[q] = (uint32_t)v;
+        keys[q + 1] = (uint32_t)curr[v];
+        vals[q + 1] = (uint32_t)v;
+    }
+}
+
+// Events sorted stably by community keep processing order inside a community, so the
+// head of each run folds localCupdate exactly as one thread's sequence of `omp atomic`
+// updates does (+= k_i on arrival, -= k_i on departure), then localCinfo += localCupdate
+// (louvain.cpp:2534-2547).
+__global__ void k_fold_events(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
+                              const int64_t* __restrict__ target, const double* __restrict__ vdeg,
+                              double* __restrict__ cdeg, int64_t* __restrict__ csize)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = keys[j];
+        if (j > 0 && keys[j - 1] == c) continue;
+        double acc = 0.0;
+        int64_t cnt = 0;
+        for (int64_t k = j; k < n && keys[k] == c; k++) {
+            const uint32_t v = vals[k];
+            if ((uint64_t)target[v] == (uint64_t)c) { acc = acc + vdeg[v]; cnt++; }
+            else { acc = acc - vdeg[v]; cnt--; }
+        }
+        cdeg[c] = cdeg[c] + acc;
+        csize[c] += cnt;
+    }
+}
+
+// currMod = |e_xx*c - a2_x*c*c| (louvain.cpp:2518-2524).
+__global__ void k_modularity(const double* __restrict__ sums, const double* __restrict__ constant,
+                             double* __restrict__ q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double c = constant[0];
+        q[0] = fabs((sums[0] * c) - (sums[1] * c * c));
+    }
+}
+
+// ET types 1/3 rotation (louvain.cpp:173-183): the freeze test parses as
+// ((target == curr) == past), a bool against a community id; vertices that fail it
+// rotate even when already frozen.
+__global__ void k_rotate_et1(int64_t nv, int num_iters, int64_t* __restrict__ past, int64_t* __restrict__ curr,
+                             int64_t* __restrict__ target, unsigned char* __restrict__ active)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = target[i], c = curr[i], p = past[i];
+        if (num_iters > 2 && (int64_t)(t == c) == p) active[i] = 0;
+        else { past[i] = c; curr[i] = t; target[i] = p; }
+    }
+}
+
+// ET types 2/4 rotation (louvain.cpp:380-400); the caller swaps p_curr/p_prev afterwards.
+__global__ void k_rotate_et2(int64_t nv, int num_iters, double one_minus_delta, int64_t* __restrict__ past,
+                             int64_t* __restrict__ curr, int64_t* __restrict__ target,
+                             unsigned char* __restrict__ active, double* __restrict__ p_curr,
+                             const double* __restrict__ p_prev)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!active[i]) continue;
+        const int64_t c = curr[i], p = past[i];
+        bool act = true;
+        if (num_iters > 2 && c == p) {
+            const double pc = p_prev[i] * one_minus_delta;
+            p_curr[i] = pc;
+            if (pc <= 0.02) { active[i] = 0; act = false; }
+        }
+        if (act) { const int64_t t = target[i]; past[i] = c; curr[i] = t; target[i] = p; }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+
+struct SweepList {
+    int64_t off, n;  // range in the list buffer
+    int bucket;
+};
+
+struct Phase {
+    vt_ctx* ctx;
+    const vt_graph* g;
+    int64_t nv;
+    bool et = false;
+    DBuf<int64_t> past, curr, target, csize;
+    DBuf<double> vdeg, cdeg, cw, frozen, pa, pb;
+    DBuf<unsigned char> active;
+    DBuf<int32_t> lists;     // vertices grouped by (color, degree bucket), vertex order inside
+    DBuf<int32_t> corder;    // vertices by color, vertex order inside (event order for color modes)
+    DBuf<double> scal;       // [0..1] sums, [2] constant, [3] q, [4] scratch
+    DBuf<int64_t> iscal;     // [0] mover total
+    DBuf<unsigned long long> ucount;
+    std::vector<std::vector<SweepList>> sweeps;  // per color (one entry in plain/vorder)
+    std::vector<int64_t> color_ptr;              // colorPtr (louvain.cpp:647-661)
+    int64_t hub_cap = 0;
+    int hub_blocks = 0;
+
+    Phase(vt_ctx* c, const vt_graph* gr) : ctx(c), g(gr), nv(gr->nv) {}
+
+    double* sums() { return scal.p; }
+    double* constant() { return scal.p + 2; }
+    double* q() { return scal.p + 3; }
+
+    void alloc(bool with_et)
+    {
+        et = with_et;
+        const size_t n = (size_t)std::max<int64_t>(nv, 1);
+        past = DBuf<int64_t>(ctx, n);
+        curr = DBuf<int64_t>(ctx, n);
+        target = DBuf<int64_t>(ctx, n);
+        csize = DBuf<int64_t>(ctx, n);
+        vdeg = DBuf<double>(ctx, n);
+        cdeg = DBuf<double>(ctx, n);
+        cw = DBuf<double>(ctx, n);
+        if (et) {
+            frozen = DBuf<double>(ctx, n);
+            pa = DBuf<double>(ctx, n);
+            pb = DBuf<double>(ctx, n);
+            active = DBuf<unsigned char>(ctx, n);
+        }
+        scal = DBuf<double>(ctx, 8);
+        iscal = DBuf<int64_t>(ctx, 4);
+        ucount = DBuf<unsigned long long>(ctx, 4);
+    }
+
+    void init(int flags)
+    {
+        VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, g->offs, g->edges, nv, vdeg.p, cdeg.p,
+                  csize.p, past.p, curr.p, target.p, et ? active.p : nullptr, et ? frozen.p : nullptr,
+                  et ? pa.p : nullptr, et ? pb.p : nullptr);
+        if (flags & VT_FLAG_SEQUENTIAL_SUMS) seq_sum2(ctx, vdeg.p, nullptr, nv, sums());
+        else tree_sum2(ctx, vdeg.p, nullptr, nv, sums());
+        VT_LAUNCH(ctx, "constant", k_constant, 1, 64, 0, (const double*)sums(), constant());
+    }
+
+    // Builds the sweep lists (and the per-color event order when colors are given).
+    void build_lists(const int64_t* d_colors, int64_t num_color)
+    {
+        const int64_t ncol = d_colors ? num_color : 1;
+        lists = DBuf<int32_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
+        uint32_t* k0 = (uint32_t*)scratch(ctx, "lk0", 4 * (size_t)std::max<int64_t>(nv, 1));
+        uint32_t* k1 = (uint32_t*)scratch(ctx, "lk1", 4 * (size_t)std::max<int64_t>(nv, 1));
+        uint32_t* v1 = (uint32_t*)scratch(ctx, "lv1", 4 * (size_t)std::max<int64_t>(nv, 1));
+        const int64_t nkeys = ncol * kNumBuckets;
+        int64_t* hist = (int64_t*)scratch(ctx, "lhist", 8 * (size_t)(nkeys + 1));
+        std::vector<int64_t> h(nkeys, 0);
+        if (nv) {
+            VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv, d_colors, num_color,
+                      k0, (uint32_t*)lists.p);
+            dzero(ctx, hist, 8 * nkeys);
+            VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(nv, 256, 8192), 256, 0, (const uint32_t*)k0, nv, hist);
+            bool alt = false;
+            sort_pairs_u32(ctx, k0, (uint32_t*)lists.p, nv, (int)bits_for(nkeys), k1, v1, &alt);
+            if (alt) d2d(ctx, lists.p, v1, 4 * (size_t)nv);
+            d2h(ctx, h.data(), hist, 8 * nkeys);
+        }
+        sweeps.assign(ncol, {});
+        int64_t off = 0;
+        for (int64_t c = 0; c < ncol; c++)
+            for (int b = 0; b < kNumBuckets; b++) {
+                const int64_t cnt = h[c * kNumBuckets + b];
+                if (cnt) sweeps[c].push_back({off, cnt, b});
+                off += cnt;
+            }
+        // Hub tables: one HBM table per resident wave, sized for the largest row.
+        hub_cap = 0;
+        if (g->max_degree > kBucketMaxDeg[kHubBucket - 1]) {
+            int64_t cap = 1;
+            while (cap < 2 * (g->max_degree + 1)) cap <<= 1;
+            hub_cap = cap;
+            int64_t nh = 0;
+            for (int64_t c = 0; c < ncol; c++)
+                for (auto& s : sweeps[c]) if (s.bucket == kHubBucket) nh = std::max(nh, s.n);
+            hub_blocks = (int)std::min<int64_t>(std::max<int64_t>(nh, 1), 1024);
+            const size_t tn = (size_t)hub_cap * hub_blocks;
+            int32_t* hk = (int32_t*)scratch(ctx, "hub_keys", 4 * tn);
+            double* hv = (double*)scratch(ctx, "hub_vals", 8 * tn);
+            scratch(ctx, "hub_occ", 4 * tn);
+            VT_LAUNCH(ctx, "hub_init", k_hbm_table_init, grid_for((int64_t)tn, 256, 8192), 256, 0, hk, hv, (int64_t)tn);
+        }
+        if (d_colors) {
+            // colorIndex: vertices by color, ascending inside a color (louvain.cpp:662-677 run
+            // by one thread).
+            corder = DBuf<int32_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
+            int64_t* ch = (int64_t*)scratch(ctx, "chist", 8 * (size_t)(ncol + 1));
+            std::vector<int64_t> hc(ncol, 0);
+            if (nv) {
+                VT_LAUNCH(ctx, "color_keys", k_color_keys, grid_for(nv, 256, 8192), 256, 0, nv, d_colors, num_color, k0,
+                          (uint32_t*)corder.p);
+                dzero(ctx, ch, 8 * ncol);
+                VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(nv, 256, 8192), 256, 0, (const uint32_t*)k0, nv, ch);
+                bool alt = false;
+                sort_pairs_u32(ctx, k0, (uint32_t*)corder.p, nv, (int)bits_for(ncol), k1, v1, &alt);
+                if (alt) d2d(ctx, corder.p, v1, 4 * (size_t)nv);
+                d2h(ctx, hc.data(), ch, 8 * ncol);
+            }
+            color_ptr.assign(ncol + 1, 0);
+            for (int64_t c = 0; c < ncol; c++) color_ptr[c + 1] = color_ptr[c] + hc[c];
+        }
+    }
+
+    MoveArgs move_args()
+    {
+        MoveArgs a;
+        a.offs = g->offs;
+        a.edges = g->edges;
+        a.curr = curr.p;
+        a.vdeg = vdeg.p;
+        a.cdeg = cdeg.p;
+        a.csize = csize.p;
+        a.active = et ? active.p : nullptr;
+        a.frozen = et ? frozen.p : nullptr;
+        a.target = target.p;
+        a.cw = cw.p;
+        a.constant = constant();
+        a.inactive_count = ucount.p;
+        return a;
+    }
+
+    template <int G, int CAP, int BLOCK>
+    void launch_lds(const MoveArgs& a, const int32_t* list, int64_t n)
+    {
+        constexpr int NG = BLOCK / G;
+        const int64_t blocks = std::min<int64_t>((n + NG - 1) / NG, 2048 * 4);
+        auto kern = k_move_lds<G, CAP, BLOCK>;
+        VT_LAUNCH(ctx, "local_move", kern, (unsigned)blocks, BLOCK, 0, a, list, n);
+    }
+
+    void sweep_lists(const std::vector<SweepList>& sl)
+    {
+        const MoveArgs a = move_args();
+        for (const auto& s : sl) {
+            const int32_t* list = lists.p + s.off;
+            switch (s.bucket) {
+                case 0: launch_lds<4, 16, 256>(a, list, s.n); break;
+                case 1: launch_lds<8, 32, 256>(a, list, s.n); break;
+                case 2: launch_lds<16, 64, 256>(a, list, s.n); break;
+                case 3: launch_lds<32, 128, 256>(a, list, s.n); break;
+                case 4: launch_lds<64, 256, 256>(a, list, s.n); break;
+                case 5: launch_lds<64, 2048, 64>(a, list, s.n); break;
+                default: {
+                    const int blocks = (int)std::min<int64_t>(s.n, hub_blocks);
+                    VT_LAUNCH(ctx, "local_move_hub", k_move_hbm, (unsigned)blocks, 64, 0, a, list, s.n,
+                              (int32_t*)scratch(ctx, "hub_keys", 0), (double*)scratch(ctx, "hub_vals", 0),
+                              (int32_t*)scratch(ctx, "hub_occ", 0), (uint32_t)hub_cap);
+                }
+            }
+        }
+    }
+
+    // Folds the sweep's moves into localCinfo in processing order; returns the mover count.
+    int64_t apply_moves(const int32_t* order, int64_t n, double* dst_cdeg, int64_t* dst_csize)
+    {
+        if (n == 0) return 0;
+        int32_t* flags = (int32_t*)scratch(ctx, "mv_flags", 4 * (size_t)n);
+        int32_t* pos = (int32_t*)scratch(ctx, "mv_pos", 4 * (size_t)n);
+        VT_LAUNCH(ctx, "mover_flags", k_mover_flags, grid_for(n, 256, 8192), 256, 0, order, n, target.p, curr.p,
+                  et ? active.p : nullptr, flags);
+        exclusive_scan_i32(ctx, flags, pos, n);
+        VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)flags, (const int32_t*)pos, n, iscal.p);
+        int64_t movers = 0;
+        d2h(ctx, &movers, iscal.p, 8);
+        if (movers == 0) return 0;
+        const int64_t ne = 2 * movers;
+        uint32_t* k0 = (uint32_t*)scratch(ctx, "ev_k0", 4 * (size_t)ne);
+        uint32_t* v0 = (uint32_t*)scratch(ctx, "ev_v0", 4 * (size_t)ne);
+        uint32_t* k1 = (uint32_t*)scratch(ctx, "ev_k1", 4 * (size_t)ne);
+        uint32_t* v1 = (uint32_t*)scratch(ctx, "ev_v1", 4 * (size_t)ne);
+        VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(n, 256, 8192), 256, 0, order, n, (const int32_t*)flags,
+                  (const int32_t*)pos, target.p, curr.p, k0, v0);
+        bool alt = false;
+        sort_pairs_u32(ctx, k0, v0, ne, (int)bits_for(nv > 1 ? nv - 1 : 1), k1, v1, &alt);
+        const uint32_t* ks = alt ? k1 : k0;
+        const uint32_t* vs = alt ? v1 : v0;
+        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, target.p, vdeg.p,
+                  dst_cdeg, dst_csize);
+        return movers;
+    }
+
+    double modularity(int flags)
+    {
+        if (flags & VT_FLAG_SEQUENTIAL_SUMS) seq_sum2(ctx, cw.p, cdeg.p, nv, sums());
+        else tree_sum2(ctx, cw.p, cdeg.p, nv, sums());
+        VT_LAUNCH(ctx, "modularity", k_modularity, 1, 64, 0, (const double*)sums(), (const double*)constant(), q());
+        double qh = 0.0;
+        d2h(ctx, &qh, q(), 8);
+        return qh;
+    }
+};
+
+double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out, int32_t* iters_out)
+{
+    int mode = p->mode, et = p->et_type;
+    if (mode < 0 || mode > 2) fail(ctx, VT_ERR_ARG, "vt_louvain_phase: bad mode %d", mode);
+    if (et < 0 || et > 4) fail(ctx, VT_ERR_ARG, "vt_louvain_phase: et_type must be 0..4 (got %d)", et);
+    if (mode != VT_MODE_PLAIN) {
+        if (p->num_color < 1) fail(ctx, VT_ERR_ARG, "vt_louvain_phase: num_color must be >= 1");
+        if (p->num_color == 1) { mode = VT_MODE_PLAIN; et = 0; }  // louvain.cpp:601-605
+        else if (!p->colors) fail(ctx, VT_ERR_ARG, "vt_louvain_phase: colors required for mode %d", mode);
+    }
+    const int64_t nv = g->nv;
+    Phase ph(ctx, g);
+    ph.alloc(et != 0);
+    ph.init(p->flags);
+
+    DBuf<int64_t> dcolors;
+    if (mode != VT_MODE_PLAIN) {
+        std::vector<int64_t> hc(p->colors, p->colors + nv);
+        for (int64_t i = 0; i < nv; i++)
+            if (hc[i] > p->num_color - 1 || hc[i] < -1)
+                fail(ctx, VT_ERR_ARG, "vt_louvain_phase: color %lld of vertex %lld outside [-1, num_color-1]",
+                     (long long)hc[i], (long long)i);
+        dcolors = DBuf<int64_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
+        h2d(ctx, dcolors.p, hc.data(), 8 * (size_t)nv);
+    }
+    // Sweep lists: per color for COLOR; a single Jacobi sweep for PLAIN and VORDER.
+    ph.build_lists(mode == VT_MODE_COLOR ? dcolors.p : nullptr, mode == VT_MODE_COLOR ? p->num_color : 1);
+    if (mode == VT_MODE_VORDER) {
+        // event order = colorIndex (color-major, vertex order inside)
+        Phase tmp(ctx, g);
+        tmp.build_lists(dcolors.p, p->num_color);
+        ph.corder = std::move(tmp.corder);
+        ph.color_ptr = tmp.color_ptr;
+    }
+
+    ctx->trace_q.clear();
+    ctx->trace_moves.clear();
+    double prevMod = p->lower, currMod = -1.0;
+    int numIters = 0;
+    const double one_minus_delta = 1.0 - p->et_delta;
+    for (;;) {
+        numIters++;
+        if (ph.et) dzero(ctx, ph.ucount.p, 8);
+        int64_t moves = 0;
+        if (mode == VT_MODE_COLOR) {
+            for (int64_t ci = 0; ci < p->num_color; ci++) {
+                ph.sweep_lists(ph.sweeps[ci]);
+                const int64_t a0 = ph.color_ptr[ci], a1 = ph.color_ptr[ci + 1];
+                moves += ph.apply_moves(ph.corder.p + a0, a1 - a0, ph.cdeg.p, ph.csize.p);
+            }
+        } else {
+            ph.sweep_lists(ph.sweeps[0]);
+        }
+        if (et == 3 || et == 4) {
+            unsigned long long vc = 0;
+            d2h(ctx, &vc, ph.ucount.p, 8);
+            if (vc >= 90) {  // ET_CUTOFF compared with a count (louvain.cpp:116-121)
+                currMod = -1.0;
+                ctx->trace_q.push_back(NAN);
+                ctx->trace_moves.push_back(moves);
+                break;
+            }
+        }
+        if (mode == VT_MODE_PLAIN) moves = ph.apply_moves(nullptr, nv, ph.cdeg.p, ph.csize.p);
+        else if (mode == VT_MODE_VORDER) moves = ph.apply_moves(ph.corder.p, nv, ph.cdeg.p, ph.csize.p);
+
+        currMod = ph.modularity(p->flags);
+        ctx->trace_q.push_back(currMod);
+        ctx->trace_moves.push_back(moves);
+        if ((currMod - prevMod) < p->thresh) break;
+        prevMod = currMod;
+        if (prevMod < p->lower) prevMod = p->lower;
+        if (p->max_iters > 0 && numIters >= p->max_iters) break;
+
+        if (et == 0) {
+            // past <- curr <- target <- old past (louvain.cpp:565-570), by pointer rotation.
+            std::swap(ph.past.p, ph.curr.p);
+            std::swap(ph.curr.p, ph.target.p);
+        } else if (et == 1 || et == 3) {
+            VT_LAUNCH(ctx, "rotate_et", k_rotate_et1, grid_for(nv, 256, 8192), 256, 0, nv, numIters, ph.past.p,
+                      ph.curr.p, ph.target.p, ph.active.p);
+        } else {
+            VT_LAUNCH(ctx, "rotate_et", k_rotate_et2, grid_for(nv, 256, 8192), 256, 0, nv, numIters, one_minus_delta,
+                      ph.past.p, ph.curr.p, ph.target.p, ph.active.p, ph.pa.p, (const double*)ph.pb.p);
+            std::swap(ph.pa.p, ph.pb.p);  // std::swap(p_curr, p_prev)
+        }
+    }
+
+    // cvect = pastComm (louvain.cpp:579); keep a device copy for the rebuild.
+    if (!g->part) g->part = (int64_t*)dalloc(ctx, 8 * (size_t)std::max<int64_t>(nv, 1));
+    d2d(ctx, g->part, ph.past.p, 8 * (size_t)nv);
+    if (cvect_out) d2h(ctx, cvect_out, ph.past.p, 8 * (size_t)nv);
+    sync(ctx);
+    *iters_out = numIters;
+    return prevMod;
+}
+
+}  // namespace vt
+
+using namespace vt;
+
+extern "C" {
+
+int vt_louvain_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out, int32_t* iters_out,
+                     double* q_out)
+{
+    if (!ctx || !g || !p) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        int32_t it = 0;
+        const double q = run_phase(ctx, g, p, cvect_out, &it);
+        if (iters_out) *iters_out = it;
+        if (q_out) *q_out = q;
+    });
+}
+
+int vt_last_phase_trace(const vt_ctx* ctx, int32_t cap, double* q, int64_t* moves, int32_t* n)
+{
+    if (!ctx) return VT_ERR_ARG;
+    const int32_t m = (int32_t)std::min<size_t>(ctx->trace_q.size(), (size_t)std::max(cap, 0));
+    for (int32_t i = 0; i < m; i++) {
+        if (q) q[i] = ctx->trace_q[i];
+        if (moves) moves[i] = ctx->trace_moves[i];
+    }
+    if (n) *n = (int32_t)ctx->trace_q.size();
+    return VT_OK;
+}
+
+int vt_init_degrees(vt_ctx* ctx, const vt_graph* g, int32_t flags, double* vdeg_out, double* constant_out)
+{
+    if (!ctx || !g) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        Phase ph(ctx, g);
+        ph.alloc(false);
+        ph.init(flags);
+        if (vdeg_out) d2h(ctx, vdeg_out, ph.vdeg.p, 8 * (size_t)g->nv);
+        if (constant_out) d2h(ctx, constant_out, ph.constant(), 8);
+        sync(ctx);
+    });
+}
+
+int vt_sweep(vt_ctx* ctx, const vt_graph* g, const int64_t* curr, const int64_t* csize, const double* cdeg,
+             const double* vdeg, double constant, int64_t* target_out, double* cw_out, int64_t* dsize_out,
+             double* ddeg_out)
+{
+    if (!ctx || !g || !curr || !csize || !cdeg || !vdeg) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        const int64_t nv = g->nv;
+        for (int64_t i = 0; i < nv; i++)
+            if (curr[i] < 0 || curr[i] >= nv) fail(ctx, VT_ERR_ARG, "vt_sweep: community of vertex %lld out of range", (long long)i);
+        Phase ph(ctx, g);
+        ph.alloc(false);
+        const size_t b8 = 8 * (size_t)nv;
+        h2d(ctx, ph.curr.p, curr, b8);
+        h2d(ctx, ph.csize.p, csize, b8);
+        h2d(ctx, ph.cdeg.p, cdeg, b8);
+        h2d(ctx, ph.vdeg.p, vdeg, b8);
+        h2d(ctx, ph.constant(), &constant, 8);
+        ph.build_lists(nullptr, 1);
+        ph.sweep_lists(ph.sweeps[0]);
+        if (target_out) d2h(ctx, target_out, ph.target.p, b8);
+        if (cw_out) d2h(ctx, cw_out, ph.cw.p, b8);
+        if (dsize_out || ddeg_out) {
+            DBuf<double> dd(ctx, (size_t)std::max<int64_t>(nv, 1));
+            DBuf<int64_t> ds(ctx, (size_t)std::max<int64_t>(nv, 1));
+            dzero(ctx, dd.p, b8);
+            dzero(ctx, ds.p, b8);
+            ph.apply_moves(nullptr, nv, dd.p, ds.p);
+            if (ddeg_out) d2h(ctx, ddeg_out, dd.p, b8);
+            if (dsize_out) d2h(ctx, dsize_out, ds.p, b8);
+        }
+        sync(ctx);
+    });
+}
+
+int vt_modularity(vt_ctx* ctx, int64_t nv, const double* cw, const double* cdeg, double constant, int32_t flags,
+                  double* q_out)
+{
+    if (!ctx || !cw || !cdeg || nv < 0) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        DBuf<double> a(ctx, (size_t)std::max<int64_t>(nv, 1)), b(ctx, (size_t)std::max<int64_t>(nv, 1)), s(ctx, 8);
+        h2d(ctx, a.p, cw, 8 * (size_t)nv);
+        h2d(ctx, b.p, cdeg, 8 * (size_t)nv);
+        h2d(ctx, s.p + 2, &constant, 8);
+        if (flags & VT_FLAG_SEQUENTIAL_SUMS) seq_sum2(ctx, a.p, b.p, nv, s.p);
+        else tree_sum2(ctx, a.p, b.p, nv, s.p);
+        VT_LAUNCH(ctx, "modularity", k_modularity, 1, 64, 0, (const double*)s.p, (const double*)(s.p + 2), s.p + 3);
+        double q = 0;
+        d2h(ctx, &q, s.p + 3, 8);
+        if (q_out) *q_out = q;
+        sync(ctx);
+    });
+}
+
+}  // extern "C"

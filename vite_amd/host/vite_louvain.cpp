@@ -1,0 +1,385 @@
+// Reference entry points (louvain.hpp, rebuild.hpp, coloring.hpp, distgraph.hpp) over the
+// vite_amd C ABI, at MPI world size 1.
+#include "vite_louvain.hpp"
+
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <random>
+
+#include "../../include/vite_amd.h"
+
+namespace {
+
+[[noreturn]] void fatal(const char* what, const char* detail)
+{
+    std::fprintf(stderr, "vite_amd: %s: %s\n", what, detail ? detail : "");
+    std::fflush(stderr);
+    std::exit(EXIT_FAILURE);
+}
+
+vt_ctx* g_ctx = nullptr;
+
+void check(int rc, const char* what)
+{
+    if (rc != VT_OK) fatal(what, vt_last_error(g_ctx));
+}
+
+void require_one_rank(int me, int nprocs, const char* fn)
+{
+    if (nprocs != 1 || me != 0) {
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "me=%d nprocs=%d; this build runs at world size 1", me, nprocs);
+        fatal(fn, buf);
+    }
+}
+
+// Device copies of host graphs, keyed by address and checked against a fingerprint of the
+// CSR so a reused address never aliases a different graph.
+struct Cached {
+    vt_graph* g = nullptr;
+    GraphElem nv = -1, ne = -1;
+    uint64_t fp = 0;
+};
+std::map<const DistGraph*, Cached> g_cache;
+
+uint64_t fingerprint(const Graph& g)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t x) {
+        h ^= x;
+        h *= 1099511628211ull;
+    };
+    const GraphElem nv = g.getNumVertices(), ne = g.getNumEdges();
+    const GraphElem step_v = nv > 4096 ? nv / 4096 : 1;
+    for (GraphElem i = 0; i <= nv; i += step_v) mix((uint64_t)g.edgeListIndexes[i]);
+    mix((uint64_t)g.edgeListIndexes[nv]);
+    const GraphElem step_e = ne > 4096 ? ne / 4096 : 1;
+    for (GraphElem e = 0; e < ne; e += step_e) {
+        uint64_t w;
+        std::memcpy(&w, &g.edgeList[e].weight, 8);
+        mix((uint64_t)g.edgeList[e].tail);
+        mix(w);
+    }
+    return h;
+}
+
+void forget(const DistGraph* dg)
+{
+    auto it = g_cache.find(dg);
+    if (it == g_cache.end()) return;
+    if (it->second.g) vt_graph_free(g_ctx, it->second.g);
+    g_cache.erase(it);
+}
+
+vt_graph* device_graph(const DistGraph& dg)
+{
+    vite_amd_context();
+    const Graph& g = dg.getLocalGraph();
+    const GraphElem nv = g.getNumVertices(), ne = g.getNumEdges();
+    const uint64_t fp = fingerprint(g);
+    auto it = g_cache.find(&dg);
+    if (it != g_cache.end() && it->second.nv == nv && it->second.ne == ne && it->second.fp == fp) return it->second.g;
+    forget(&dg);
+    Cached c;
+    check(vt_graph_upload(g_ctx, nv, ne, g.edgeListIndexes.data(), reinterpret_cast<const vt_edge*>(g.edgeList.data()),
+                          &c.g),
+          "graph upload");
+    c.nv = nv;
+    c.ne = ne;
+    c.fp = fp;
+    g_cache[&dg] = c;
+    return c.g;
+}
+
+// exchangeVertexReqs at world size 1 (louvain.cpp:3249-3397): no ghosts.
+void no_ghosts(int nprocs, size_t& ssz, size_t& rsz, std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+               std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata)
+{
+    ssz = rsz = 0;
+    ssizes.assign(nprocs, 0);
+    rsizes.assign(nprocs, 0);
+    svdata.clear();
+    rvdata.clear();
+}
+
+GraphWeight phase(const int me, const int nprocs, const DistGraph& dg, size_t& ssz, size_t& rsz,
+                  std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes, std::vector<GraphElem>& svdata,
+                  std::vector<GraphElem>& rvdata, CommunityVector& cvect, GraphWeight lower, GraphWeight thresh,
+                  int& iters, int mode, int et_type, GraphWeight et_delta, long numColor, const ColorVector* colors)
+{
+    require_one_rank(me, nprocs, "distLouvainMethod");
+    no_ghosts(nprocs, ssz, rsz, ssizes, rsizes, svdata, rvdata);
+    vt_graph* g = device_graph(dg);
+    vt_phase_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.lower = lower;
+    p.thresh = thresh;
+    p.et_type = et_type;
+    p.mode = mode;
+    p.et_delta = et_delta;
+    p.num_color = numColor;
+    p.colors = colors ? colors->data() : nullptr;
+    const char* seq = std::getenv("VITE_AMD_SEQUENTIAL_SUMS");
+    if (seq && seq[0] == '1') p.flags |= VT_FLAG_SEQUENTIAL_SUMS;
+    cvect.resize(dg.getLocalGraph().getNumVertices());
+    int32_t it = 0;
+    double q = 0.0;
+    check(vt_louvain_phase(g_ctx, g, &p, cvect.data(), &it, &q), "distLouvainMethod");
+    iters = it;
+    return q;
+}
+
+}  // namespace
+
+vt_ctx* vite_amd_context()
+{
+    if (!g_ctx) {
+        const char* d = std::getenv("VITE_AMD_DEVICE");
+        const int dev = d ? std::atoi(d) : 0;
+        if (vt_ctx_create(dev, 0, &g_ctx) != VT_OK) fatal("context", "no usable GPU");
+    }
+    return g_ctx;
+}
+
+void DistGraph::printStats()
+{
+    const Graph& g = getLocalGraph();
+    long ne = (long)g.getNumEdges(), nv = (long)getTotalNumVertices();
+    std::cout << std::endl
+              << "-------------------------------------------------------" << std::endl
+              << "Graph edge distribution characteristics" << std::endl
+              << "-------------------------------------------------------" << std::endl
+              << "Number of vertices: " << nv << std::endl
+              << "Number of edges: " << ne << std::endl
+              << "Maximum number of edges: " << ne << std::endl
+              << "Average number of edges: " << (double)ne << std::endl
+              << "Minimum number of edges: " << ne << std::endl
+              << "-------------------------------------------------------" << std::endl;
+}
+
+// ---- the nine Louvain overloads -----------------------------------------------------------
+
+GraphWeight distLouvainMethod(const int me, const int nprocs, const DistGraph& dg, size_t& ssz, size_t& rsz,
+                              std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                              std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata, CommunityVector& cvect,
+                              const GraphWeight lower, const GraphWeight thresh, int& iters)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_PLAIN, 0,
+                 1.0, 0, nullptr);
+}
+
+GraphWeight distLouvainMethod(const int me, const int nprocs, const DistGraph& dg, size_t& ssz, size_t& rsz,
+                              std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                              std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata, CommunityVector& cvect,
+                              const GraphWeight lower, const GraphWeight thresh, int& iters, bool ETLocalOrRemote)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_PLAIN,
+                 ETLocalOrRemote ? 1 : 3, 1.0, 0, nullptr);
+}
+
+GraphWeight distLouvainMethod(const int me, const int nprocs, const DistGraph& dg, size_t& ssz, size_t& rsz,
+                              std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                              std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata, CommunityVector& cvect,
+                              const GraphWeight lower, const GraphWeight thresh, int& iters, GraphWeight ETDelta,
+                              bool ETLocalOrRemote)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_PLAIN,
+                 ETLocalOrRemote ? 2 : 4, ETDelta, 0, nullptr);
+}
+
+GraphWeight distLouvainMethodWithColoring(const int me, const int nprocs, const DistGraph& dg, const long numColor,
+                                          const ColorVector& vertexColor, size_t& ssz, size_t& rsz,
+                                          std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                                          std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata,
+                                          CommunityVector& cvect, const GraphWeight lower, const GraphWeight thresh,
+                                          int& iters)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_COLOR, 0,
+                 1.0, numColor, &vertexColor);
+}
+
+GraphWeight distLouvainMethodWithColoring(const int me, const int nprocs, const DistGraph& dg, const long numColor,
+                                          const ColorVector& vertexColor, size_t& ssz, size_t& rsz,
+                                          std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                                          std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata,
+                                          CommunityVector& cvect, const GraphWeight lower, const GraphWeight thresh,
+                                          int& iters, bool ETLocalOrRemote)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_COLOR,
+                 ETLocalOrRemote ? 1 : 3, 1.0, numColor, &vertexColor);
+}
+
+GraphWeight distLouvainMethodWithColoring(const int me, const int nprocs, const DistGraph& dg, const long numColor,
+                                          const ColorVector& vertexColor, size_t& ssz, size_t& rsz,
+                                          std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                                          std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata,
+                                          CommunityVector& cvect, const GraphWeight lower, const GraphWeight thresh,
+                                          int& iters, GraphWeight ETDelta, bool ETLocalOrRemote)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_COLOR,
+                 ETLocalOrRemote ? 2 : 4, ETDelta, numColor, &vertexColor);
+}
+
+GraphWeight distLouvainMethodVertexOrder(const int me, const int nprocs, const DistGraph& dg, const long numColor,
+                                         const ColorVector& vertexColor, size_t& ssz, size_t& rsz,
+                                         std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                                         std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata,
+                                         CommunityVector& cvect, const GraphWeight lower, const GraphWeight thresh,
+                                         int& iters)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_VORDER, 0,
+                 1.0, numColor, &vertexColor);
+}
+
+GraphWeight distLouvainMethodVertexOrder(const int me, const int nprocs, const DistGraph& dg, const long numColor,
+                                         const ColorVector& vertexColor, size_t& ssz, size_t& rsz,
+                                         std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                                         std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata,
+                                         CommunityVector& cvect, const GraphWeight lower, const GraphWeight thresh,
+                                         int& iters, bool ETLocalOrRemote)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_VORDER,
+                 ETLocalOrRemote ? 1 : 3, 1.0, numColor, &vertexColor);
+}
+
+GraphWeight distLouvainMethodVertexOrder(const int me, const int nprocs, const DistGraph& dg, const long numColor,
+                                         const ColorVector& vertexColor, size_t& ssz, size_t& rsz,
+                                         std::vector<GraphElem>& ssizes, std::vector<GraphElem>& rsizes,
+                                         std::vector<GraphElem>& svdata, std::vector<GraphElem>& rvdata,
+                                         CommunityVector& cvect, const GraphWeight lower, const GraphWeight thresh,
+                                         int& iters, GraphWeight ETDelta, bool ETLocalOrRemote)
+{
+    return phase(me, nprocs, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect, lower, thresh, iters, VT_MODE_VORDER,
+                 ETLocalOrRemote ? 2 : 4, ETDelta, numColor, &vertexColor);
+}
+
+// ---- rebuild / coloring ---------------------------------------------------------------------
+
+static DistGraph* host_from_device(vt_graph* g)
+{
+    int64_t nv = 0, ne = 0;
+    vt_graph_info(g, &nv, &ne);
+    DistGraph* dg = new DistGraph(nv, ne);
+    PartRanges parts{0, nv};
+    dg->createLocalGraph(nv, ne, &parts);
+    Graph& lg = dg->getLocalGraph();
+    check(vt_graph_download(g_ctx, g, lg.edgeListIndexes.data(), reinterpret_cast<vt_edge*>(lg.edgeList.data())),
+          "graph download");
+    Cached c;
+    c.g = g;
+    c.nv = nv;
+    c.ne = ne;
+    c.fp = fingerprint(lg);
+    g_cache[dg] = c;
+    return dg;
+}
+
+void distbuildNextLevelGraph(int nprocs, int me, DistGraph*& dg, const size_t&, const size_t&,
+                             const std::vector<GraphElem>&, const std::vector<GraphElem>&,
+                             const std::vector<GraphElem>&, const std::vector<GraphElem>&, CommunityVector& cvect)
+{
+    require_one_rank(me, nprocs, "distbuildNextLevelGraph");
+    vt_graph* g = device_graph(*dg);
+    if ((GraphElem)cvect.size() != dg->getLocalGraph().getNumVertices())
+        fatal("distbuildNextLevelGraph", "cvect size differs from the number of vertices");
+    vt_graph* out = nullptr;
+    check(vt_rebuild(g_ctx, g, cvect.data(), &out), "distbuildNextLevelGraph");
+    forget(dg);
+    delete dg;  // the reference replaces *dg (rebuild.cpp:334,351)
+    dg = host_from_device(out);
+}
+
+ColorElem distColoringMultiHashMinMax(const int me, const int nprocs, const DistGraph& dg, ColorVector& vertexColor,
+                                      const ColorElem nHash, const int target_percent, const bool singleIteration)
+{
+    require_one_rank(me, nprocs, "distColoringMultiHashMinMax");
+    vt_graph* g = device_graph(dg);
+    vertexColor.resize(dg.getLocalGraph().getNumVertices());
+    int64_t nc = 0;
+    check(vt_color(g_ctx, g, nHash, target_percent, singleIteration ? 1 : 0, vertexColor.data(), &nc),
+          "distColoringMultiHashMinMax");
+    return nc;
+}
+
+// ---- graph input / generation ---------------------------------------------------------------
+
+void loadDistGraphMPIIO(int me, int nprocs, int, DistGraph*& dg, std::string& fileName)
+{
+    require_one_rank(me, nprocs, "loadDistGraphMPIIO");
+    std::ifstream f(fileName, std::ios::binary);
+    if (!f) fatal("loadDistGraphMPIIO", (" Error opening file! " + fileName).c_str());
+    GraphElem hdr[2];
+    f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+    if (!f || hdr[0] < 0 || hdr[1] < 0) fatal("loadDistGraphMPIIO", "bad header");
+    std::cout << "Reading file of " << hdr[0] << " vertices and " << hdr[1] << " edges." << std::endl;
+    dg = new DistGraph(hdr[0], hdr[1]);
+    PartRanges parts{0, hdr[0]};
+    dg->createLocalGraph(hdr[0], hdr[1], &parts);
+    Graph& g = dg->getLocalGraph();
+    f.read(reinterpret_cast<char*>(g.edgeListIndexes.data()), sizeof(GraphElem) * (hdr[0] + 1));
+    f.read(reinterpret_cast<char*>(g.edgeList.data()), sizeof(Edge) * hdr[1]);
+    if (!f) fatal("loadDistGraphMPIIO", "short read");
+    const GraphElem base = g.edgeListIndexes[0];  // rebase (distgraph.cpp:244-247)
+    for (auto& o : g.edgeListIndexes) o -= base;
+}
+
+void writeGraph(const DistGraph& dg, const std::string& fileName)
+{
+    const Graph& g = dg.getLocalGraph();
+    std::ofstream f(fileName, std::ios::binary);
+    if (!f) fatal("writeGraph", (" Error opening output binary file for storing graph data! " + fileName).c_str());
+    GraphElem hdr[2] = {g.getNumVertices(), g.getNumEdges()};
+    f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+    f.write(reinterpret_cast<const char*>(g.edgeListIndexes.data()), sizeof(GraphElem) * (hdr[0] + 1));
+    f.write(reinterpret_cast<const char*>(g.edgeList.data()), sizeof(Edge) * hdr[1]);
+}
+
+extern "C" int vt_gen_vite_rgg(vt_ctx* ctx, int64_t n, uint32_t x0, vt_graph** out);
+
+// generateInMemGraph at one rank (distgraph.cpp:391-407): Vite's LCG-seeded RGG, built on the
+// device.  x0 is std::seed_seq{1}'s first word, as LCG::reseeder draws it (utils.hpp:163-170).
+void generateInMemGraph(int rank, int nprocs, DistGraph*& dg, GraphElem nv, GraphWeight randomEdgePercent,
+                        std::string fileOut)
+{
+    require_one_rank(rank, nprocs, "generateInMemGraph");
+    if (randomEdgePercent > 0.0)
+        fatal("generateInMemGraph", "random extra edges use a time seed in the reference; use generateSeededRGG");
+    vite_amd_context();
+    std::seed_seq seq({1u});
+    std::vector<std::uint32_t> seeds(1);
+    seq.generate(seeds.begin(), seeds.end());
+    vt_graph* g = nullptr;
+    check(vt_gen_vite_rgg(g_ctx, nv, seeds[0], &g), "generateInMemGraph");
+    dg = host_from_device(g);
+    if (!fileOut.empty()) {
+        writeGraph(*dg, fileOut);
+        std::cout << "Written binary file: " << fileOut << std::endl;
+    }
+}
+
+DistGraph* generateSeededRGG(unsigned long long seed, GraphElem n, GraphWeight extraPercent)
+{
+    vite_amd_context();
+    vt_graph* g = nullptr;
+    check(vt_gen_rgg(g_ctx, seed, n, extraPercent, &g), "generateSeededRGG");
+    return host_from_device(g);
+}
+
+DistGraph* generateSeededBA(unsigned long long seed, GraphElem n, GraphElem m)
+{
+    vite_amd_context();
+    vt_graph* g = nullptr;
+    check(vt_gen_ba(g_ctx, seed, n, m, &g), "generateSeededBA");
+    return host_from_device(g);
+}
+
+void createCommunityMPIType() {}
+void destroyCommunityMPIType() {}
+void createEdgeMPIType() {}
+void destroyEdgeMPIType() {}
