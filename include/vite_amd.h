@@ -100,6 +100,23 @@ typedef struct {
 int vt_louvain_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out,
                      int32_t* iters_out, double* q_out);
 
+/* The same phase, one iteration at a time, with its state resident on the device between
+ * calls (the locals of distLouvainMethod*, louvain.cpp:14-50).  vt_phase_step runs one
+ * iteration of the reference loop body and reports whether the reference would stop;
+ * vt_phase_sweep runs only the local move over every vertex from the current state without
+ * applying the moves (repeatable: it leaves the state unchanged); vt_phase_end returns what
+ * vt_louvain_phase returns and frees the state. */
+typedef struct vt_phase vt_phase;
+int vt_phase_begin(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, vt_phase** out);
+int vt_phase_step(vt_ctx* ctx, vt_phase* st, double* q_out, int32_t* done_out);
+int vt_phase_sweep(vt_ctx* ctx, vt_phase* st);
+int vt_phase_end(vt_ctx* ctx, vt_phase* st, int64_t* cvect_out, int32_t* iters_out, double* q_out);
+/* Copies the current state to host arrays (nv entries each; any may be NULL): currComm,
+ * localCinfo sizes and degrees, vertex degrees, 1/2m, and the last sweep's targetComm and
+ * clusterWeight. */
+int vt_phase_get_state(vt_ctx* ctx, const vt_phase* st, int64_t* curr, int64_t* csize, double* cdeg, double* vdeg,
+                       double* constant, int64_t* target, double* cw);
+
 /* Per-iteration record of the last vt_louvain_phase call on this context. */
 int vt_last_phase_trace(const vt_ctx* ctx, int32_t cap, double* q, int64_t* moves, int32_t* n);
 

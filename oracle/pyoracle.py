@@ -109,6 +109,7 @@ def _ref():
         h.ref_init_degrees.restype = C.c_double
         h.ref_modularity.restype = C.c_double
         h.ref_color.restype = C.c_long
+        h.ref_sweep_omp.restype = C.c_double
         _ref.h = h
     return _ref.h
 
@@ -331,6 +332,20 @@ def ref_sweep(g: Graph, curr, csize, cdeg, vdeg, constant, order=None):
     mask[order] = False
     target[mask] = curr[mask]
     return target, cw, dsize, ddeg
+
+
+def ref_sweep_omp(g: Graph, curr, csize, cdeg, vdeg, constant, v0: int, v1: int, threads: int, reps: int = 1):
+    """The reference's parallel sweep loop on vertices [v0, v1): (mean seconds, targets)."""
+    nv = g.nv
+    curr = np.ascontiguousarray(curr, np.int64)
+    csize = np.ascontiguousarray(csize, np.int64)
+    cdeg = np.ascontiguousarray(cdeg, np.float64)
+    vdeg = np.ascontiguousarray(vdeg, np.float64)
+    target = np.zeros(nv, np.int64)
+    sec = _ref().ref_sweep_omp(C.c_int64(nv), C.c_int64(g.ne), _p(g.offs), g.edges.ctypes.data_as(C.c_void_p),
+                               _p(curr), _p(csize), _p(cdeg, f64p), _p(vdeg, f64p), C.c_double(constant),
+                               C.c_int64(v0), C.c_int64(v1), C.c_int(threads), C.c_int(reps), _p(target))
+    return sec, target
 
 
 def ref_init_degrees(g: Graph):

@@ -221,6 +221,43 @@ void ref_sweep(std::int64_t nv, std::int64_t ne, const std::int64_t* offs, const
     delete dg;
 }
 
+// The reference's sweep loop (louvain.cpp:497-514: omp parallel, distCleanCWandCU, omp for
+// schedule(guided) over distExecuteLouvainIteration) on vertices [v0, v1) with `nthreads`
+// threads, `reps` times from the same state.  Returns the mean seconds of the loop alone;
+// targets of the last repetition go to target_out (nv entries; untouched vertices get -1).
+double ref_sweep_omp(std::int64_t nv, std::int64_t ne, const std::int64_t* offs, const Edge* edges,
+                     const std::int64_t* curr, const std::int64_t* csize, const double* cdeg, const double* vdeg,
+                     double constant, std::int64_t v0, std::int64_t v1, int nthreads, int reps,
+                     std::int64_t* target_out)
+{
+    ensure_mpi();
+    DistGraph* dg = make_dg(nv, ne, offs, edges);
+    CommunityVector currComm(curr, curr + nv), targetComm(nv, -1);
+    GraphWeightVector vDegree(vdeg, vdeg + nv), clusterWeight(nv, 0.0);
+    CommVector localCinfo(nv), localCupdate(nv);
+    for (std::int64_t i = 0; i < nv; i++) { localCinfo[i].size = csize[i]; localCinfo[i].degree = cdeg[i]; }
+    VertexCommMap remoteComm;
+    CommMap remoteCinfo, remoteCupdate;
+    omp_set_num_threads(nthreads);
+    double total = 0.0;
+    for (int r = 0; r < reps; r++) {
+        const double t0 = omp_get_wtime();
+#pragma omp parallel shared(clusterWeight, localCupdate, currComm, targetComm, vDegree, localCinfo, remoteCinfo, \
+                                remoteComm, dg, remoteCupdate) firstprivate(constant)
+        {
+            distCleanCWandCU(nv, clusterWeight, localCupdate);
+#pragma omp for schedule(guided)
+            for (std::int64_t i = v0; i < v1; i++)
+                distExecuteLouvainIteration(i, *dg, currComm, targetComm, vDegree, localCinfo, localCupdate,
+                                            remoteComm, remoteCinfo, remoteCupdate, constant, clusterWeight, 0);
+        }
+        total += omp_get_wtime() - t0;
+    }
+    for (std::int64_t i = 0; i < nv; i++) target_out[i] = targetComm[i];
+    delete dg;
+    return total / (reps > 0 ? reps : 1);
+}
+
 // distInitLouvain pieces (louvain.cpp:2161-2218): vertex degrees and 1/2m.
 double ref_init_degrees(std::int64_t nv, std::int64_t ne, const std::int64_t* offs, const Edge* edges,
                         double* vdeg_out)

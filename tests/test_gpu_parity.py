@@ -175,3 +175,39 @@ def test_run_tree_sums(ctx, name):
     r_o = po.run(g, threshold_cycling=True)
     r = dg.run(threshold_cycling=True)
     assert abs(r.final_q - r_o.final_q) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["rgg", "ba", "hub", "multi"])
+@pytest.mark.parametrize("et", [0, 1, 2])
+def test_phase_stepwise_equals_oneshot(ctx, name, et):
+    """vt_phase_begin/step/end reproduce vt_louvain_phase exactly."""
+    g, dg = _upload(ctx, name)
+    q1, cv1, it1 = dg.louvain_phase(et_type=et, et_delta=0.4)
+    st = dg.phase_begin(et_type=et, et_delta=0.4)
+    done = False
+    while not done:
+        _, done = st.step()
+    q2, cv2, it2 = st.end()
+    assert (q1, it1) == (q2, it2)
+    assert np.array_equal(cv1, cv2)
+
+
+@pytest.mark.parametrize("name", ["rgg", "ba", "hub", "vite_rgg"])
+@pytest.mark.parametrize("prep", [0, 1, 3])
+def test_phase_sweep_state_replay(ctx, name, prep):
+    """A sweep from a device mid-run state equals the oracle's sweep from the downloaded
+    state, and repeating it changes nothing."""
+    g, dg = _upload(ctx, name)
+    st = dg.phase_begin(thresh=-1.0)
+    for _ in range(prep):
+        st.step()
+    st.sweep()
+    s1 = st.state()
+    st.sweep()
+    s2 = st.state()
+    assert np.array_equal(s1["target"], s2["target"])
+    assert np.array_equal(s1["cw"].view(np.uint64), s2["cw"].view(np.uint64))
+    t_o, cw_o, _, _ = po.sweep(g, s1["curr"], s1["csize"], s1["cdeg"], s1["vdeg"], s1["constant"])
+    assert np.array_equal(s1["target"], t_o)
+    assert np.array_equal(s1["cw"].view(np.uint64), cw_o.view(np.uint64))
+    st.end(want_cvect=False)

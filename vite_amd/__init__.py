@@ -81,6 +81,11 @@ _SIGS = {
     "vt_louvain_phase": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_PhaseParams), _i64p,
                                    C.POINTER(C.c_int32), _f64p]),
     "vt_last_phase_trace": (C.c_int, [C.c_void_p, C.c_int32, _f64p, _i64p, C.POINTER(C.c_int32)]),
+    "vt_phase_begin": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_PhaseParams), C.POINTER(C.c_void_p)]),
+    "vt_phase_step": (C.c_int, [C.c_void_p, C.c_void_p, _f64p, C.POINTER(C.c_int32)]),
+    "vt_phase_sweep": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vt_phase_end": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.POINTER(C.c_int32), _f64p]),
+    "vt_phase_get_state": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, _i64p, _f64p, _f64p, _f64p, _i64p, _f64p]),
     "vt_sweep": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, _i64p, _f64p, _f64p, C.c_double, _i64p, _f64p,
                            _i64p, _f64p]),
     "vt_modularity": (C.c_int, [C.c_void_p, C.c_int64, _f64p, _f64p, C.c_double, C.c_int32, _f64p]),
@@ -264,6 +269,18 @@ class DeviceGraph:
                                              C.byref(it), C.byref(q)))
         return q.value, cvect, it.value
 
+    def phase_begin(self, lower=-1.0, thresh=1e-6, et_type=0, et_delta=1.0, mode="plain", num_color=0, colors=None,
+                    sequential=False, max_iters=0) -> "PhaseState":
+        m = {"plain": VT_MODE_PLAIN, "color": VT_MODE_COLOR, "vorder": VT_MODE_VORDER}[mode]
+        p = _PhaseParams(lower, thresh, et_type, m, et_delta, num_color, None,
+                         VT_FLAG_SEQUENTIAL_SUMS if sequential else 0, max_iters)
+        if colors is not None:
+            colors = np.ascontiguousarray(colors, np.int64)
+            p.colors = _p(colors)
+        h = C.c_void_p()
+        self.ctx.check(_lib.vt_phase_begin(self.ctx.h, self.h, C.byref(p), C.byref(h)))
+        return PhaseState(self, h)
+
     def last_trace(self, cap: int = 10000):
         q = np.zeros(cap)
         mv = np.zeros(cap, np.int64)
@@ -312,6 +329,46 @@ class DeviceGraph:
         return RunResult(list(res.level_q[:n]), list(res.level_iters[:n]), list(res.level_nv[:n]),
                          list(res.level_ne[:n]), list(res.call_q[:m]), list(res.call_iters[:m]), res.final_q,
                          res.total_iters, res.num_colors, res.clustering_seconds, res.total_seconds)
+
+
+class PhaseState:
+    """A Louvain phase advanced one iteration at a time, state resident on the device."""
+
+    def __init__(self, g: DeviceGraph, h):
+        self.g = g
+        self.ctx = g.ctx
+        self.h = h
+
+    def step(self):
+        q, done = C.c_double(), C.c_int32()
+        self.ctx.check(_lib.vt_phase_step(self.ctx.h, self.h, C.byref(q), C.byref(done)))
+        return q.value, bool(done.value)
+
+    def sweep(self):
+        self.ctx.check(_lib.vt_phase_sweep(self.ctx.h, self.h))
+
+    def state(self):
+        nv = self.g.nv
+        curr, csize, target = np.zeros(nv, np.int64), np.zeros(nv, np.int64), np.zeros(nv, np.int64)
+        cdeg, vdeg, cw = np.zeros(nv), np.zeros(nv), np.zeros(nv)
+        c = C.c_double()
+        self.ctx.check(_lib.vt_phase_get_state(self.ctx.h, self.h, _p(curr), _p(csize), _p(cdeg, _f64p),
+                                               _p(vdeg, _f64p), C.byref(c), _p(target), _p(cw, _f64p)))
+        return dict(curr=curr, csize=csize, cdeg=cdeg, vdeg=vdeg, constant=c.value, target=target, cw=cw)
+
+    def end(self, want_cvect=True):
+        cvect = np.zeros(self.g.nv, np.int64) if want_cvect else None
+        it, q = C.c_int32(), C.c_double()
+        h, self.h = self.h, None
+        self.ctx.check(_lib.vt_phase_end(self.ctx.h, h, _p(cvect) if want_cvect else None, C.byref(it), C.byref(q)))
+        return q.value, cvect, it.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.end(want_cvect=False)
+        except Exception:
+            pass
 
 
 @dataclass

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -47,7 +48,6 @@ __device__ __forceinline__ uint32_t slot_hash(int32_t c) { return (uint32_t)c * 
 // Degree buckets: (lanes per vertex G, table capacity CAP, max degree).  CAP >= 2*(deg+1)
 // keeps the table at most half full (a row has at most deg+1 distinct communities).
 constexpr int kNumBuckets = 7;
-constexpr int64_t kBucketMaxDeg[kNumBuckets] = {7, 15, 31, 63, 127, 1023, INT64_MAX};
 constexpr int kHubBucket = 6;
 
 __host__ __device__ inline int degree_bucket(int64_t d)
@@ -284,38 +284,95 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
     wave_sync();
 }
 
-// LDS-table kernel: BLOCK threads, BLOCK/G groups, each with its own CAP-slot table.
-template <int G, int CAP, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_move_lds(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
-{
-    using mask_t = typename MaskT<G>::type;
-    constexpr int NG = BLOCK / G;
-    __shared__ int32_t s_keys[NG][CAP];
-    __shared__ double s_vals[NG][CAP];
-    __shared__ mask_t s_masks[NG][CAP];
-    __shared__ int32_t s_occ[NG][CAP];
-    __shared__ double s_wst[NG][G];
-    __shared__ int32_t s_nocc[NG], s_cslot[NG];
-    __shared__ unsigned s_inactive;
+// LDS carve-out for BLOCK/G groups of CAP-slot tables.
+constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+template <int G, int CAP, int BLOCK>
+struct LdsLayout {
+    using mask_t = typename MaskT<G>::type;
+    static constexpr int NG = BLOCK / G;
+    static constexpr size_t keys = 0;
+    static constexpr size_t vals = align16(keys + sizeof(int32_t) * NG * CAP);
+    static constexpr size_t masks = align16(vals + sizeof(double) * NG * CAP);
+    static constexpr size_t occ = align16(masks + sizeof(mask_t) * NG * CAP);
+    static constexpr size_t wst = align16(occ + sizeof(int32_t) * NG * CAP);
+    static constexpr size_t nocc = align16(wst + sizeof(double) * NG * G);
+    static constexpr size_t cslot = align16(nocc + sizeof(int32_t) * NG);
+    static constexpr size_t bytes = align16(cslot + sizeof(int32_t) * NG);
+};
+
+// Runs one degree bucket inside a block: block `blk` of `nblk` owns list entries
+// blk*NG + g, stepping by nblk*NG.  Every thread of the block must call it.
+template <int G, int CAP, int BLOCK>
+__device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const int32_t* __restrict__ list, int64_t n,
+                                           int64_t blk, int64_t nblk, unsigned* inactive)
+{
+    using L = LdsLayout<G, CAP, BLOCK>;
+    using mask_t = typename L::mask_t;
+    constexpr int NG = L::NG;
+    int32_t* keys = (int32_t*)(lds + L::keys);
+    double* vals = (double*)(lds + L::vals);
+    mask_t* masks = (mask_t*)(lds + L::masks);
+    int32_t* occ = (int32_t*)(lds + L::occ);
+    double* wst = (double*)(lds + L::wst);
+    int32_t* nocc = (int32_t*)(lds + L::nocc);
+    int32_t* cslot = (int32_t*)(lds + L::cslot);
+    for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
+        keys[i] = -1;
+        vals[i] = sent_value();
+        masks[i] = 0;
+    }
+    if (threadIdx.x < NG) nocc[threadIdx.x] = 0;
+    __syncthreads();
     const int g = threadIdx.x / G;
     const int lane = threadIdx.x % G;
     const int gbase = (threadIdx.x % 64) / G * G;
-    for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
-        (&s_keys[0][0])[i] = -1;
-        (&s_vals[0][0])[i] = sent_value();
-        (&s_masks[0][0])[i] = 0;
-    }
-    if (threadIdx.x < NG) s_nocc[threadIdx.x] = 0;
+    Table<G, false> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, occ + g * CAP, wst + g * G, nullptr,
+                      nocc + g, cslot + g, (uint32_t)(CAP - 1)};
+    for (int64_t idx = blk * NG + g; idx < n; idx += nblk * NG) move_vertex<G, false>(a, t, list[idx], lane, gbase, inactive);
+}
+
+// Buckets 0-4 (degree <= 127) of one sweep in a single launch: blocks [bbeg[b], bbeg[b+1])
+// run bucket b over its list range.
+constexpr int kFusedBuckets = 5;
+constexpr int kFusedBlock = 256;
+constexpr size_t kFusedLds = std::max({LdsLayout<4, 16, 256>::bytes, LdsLayout<8, 32, 256>::bytes,
+                                       LdsLayout<16, 64, 256>::bytes, LdsLayout<32, 128, 256>::bytes,
+                                       LdsLayout<64, 256, 256>::bytes});
+
+struct SweepPlan {
+    int64_t bbeg[kFusedBuckets + 1];  // block ranges
+    int64_t off[kFusedBuckets];       // list offsets
+    int64_t n[kFusedBuckets];         // list lengths
+};
+
+__global__ void __launch_bounds__(kFusedBlock) k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
+{
+    __shared__ __align__(16) char lds[kFusedLds];
+    __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
+    int b = 0;
+    while (b < kFusedBuckets - 1 && (int64_t)blockIdx.x >= plan.bbeg[b + 1]) b++;
+    const int64_t blk = blockIdx.x - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
+    const int32_t* list = lists + plan.off[b];
+    switch (b) {
+        case 0: run_bucket<4, 16, 256>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
+        case 1: run_bucket<8, 32, 256>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+        case 2: run_bucket<16, 64, 256>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
+        case 3: run_bucket<32, 128, 256>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+        default: run_bucket<64, 256, 256>(a, lds, list, plan.n[4], blk, nblk, &s_inactive); break;
+    }
     __syncthreads();
+    if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
+}
 
-    Table<G, false> t{s_keys[g], s_vals[g], s_masks[g], s_occ[g], s_wst[g], nullptr, &s_nocc[g], &s_cslot[g],
-                      (uint32_t)(CAP - 1)};
-    const int64_t stride = (int64_t)gridDim.x * NG;
-    for (int64_t idx = (int64_t)blockIdx.x * NG + g; idx < n; idx += stride)
-        move_vertex<G, false>(a, t, list[idx], lane, gbase, &s_inactive);
-
+// Bucket 5 (128 <= degree <= 1023): one wave per vertex, 2048-slot LDS table.
+__global__ void __launch_bounds__(64) k_local_move_wide(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
+{
+    __shared__ __align__(16) char lds[LdsLayout<64, 2048, 64>::bytes];
+    __shared__ unsigned s_inactive;
+    if (threadIdx.x == 0) s_inactive = 0;
+    run_bucket<64, 2048, 64>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
@@ -514,9 +571,11 @@ __global__ void k_rotate_et2(int64_t nv, int num_iters, double one_minus_delta, 
 // Host side
 // ------------------------------------------------------------------------------------------
 
-struct SweepList {
-    int64_t off, n;  // range in the list buffer
-    int bucket;
+// One color's sweep: buckets 0-4 fused into one launch, then the wide and hub buckets.
+struct ColorSweep {
+    SweepPlan plan{};
+    int64_t fused_blocks = 0;
+    int64_t wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0;
 };
 
 struct Phase {
@@ -529,11 +588,11 @@ struct Phase {
     DBuf<unsigned char> active;
     DBuf<int32_t> lists;     // vertices grouped by (color, degree bucket), vertex order inside
     DBuf<int32_t> corder;    // vertices by color, vertex order inside (event order for color modes)
-    DBuf<double> scal;       // [0..1] sums, [2] constant, [3] q, [4] scratch
+    DBuf<double> scal;       // [0..1] sums, [2] constant, [3] q
     DBuf<int64_t> iscal;     // [0] mover total
     DBuf<unsigned long long> ucount;
-    std::vector<std::vector<SweepList>> sweeps;  // per color (one entry in plain/vorder)
-    std::vector<int64_t> color_ptr;              // colorPtr (louvain.cpp:647-661)
+    std::vector<ColorSweep> sweeps;      // per color (one entry in plain/vorder)
+    std::vector<int64_t> color_ptr;      // colorPtr (louvain.cpp:647-661)
     int64_t hub_cap = 0;
     int hub_blocks = 0;
 
@@ -575,20 +634,22 @@ struct Phase {
         VT_LAUNCH(ctx, "constant", k_constant, 1, 64, 0, (const double*)sums(), constant());
     }
 
-    // Builds the sweep lists (and the per-color event order when colors are given).
-    void build_lists(const int64_t* d_colors, int64_t num_color)
+    // Sorts vertices into (color, degree bucket) lists and, with colors, the per-color event
+    // order; per color, plans one fused launch for buckets 0-4.
+    void build_lists(const int64_t* d_colors, int64_t num_color, bool want_corder)
     {
-        const int64_t ncol = d_colors ? num_color : 1;
-        lists = DBuf<int32_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
-        uint32_t* k0 = (uint32_t*)scratch(ctx, "lk0", 4 * (size_t)std::max<int64_t>(nv, 1));
-        uint32_t* k1 = (uint32_t*)scratch(ctx, "lk1", 4 * (size_t)std::max<int64_t>(nv, 1));
-        uint32_t* v1 = (uint32_t*)scratch(ctx, "lv1", 4 * (size_t)std::max<int64_t>(nv, 1));
+        const int64_t ncol = d_colors && !want_corder ? num_color : 1;
+        const size_t n1 = (size_t)std::max<int64_t>(nv, 1);
+        lists = DBuf<int32_t>(ctx, n1);
+        uint32_t* k0 = (uint32_t*)scratch(ctx, "lk0", 4 * n1);
+        uint32_t* k1 = (uint32_t*)scratch(ctx, "lk1", 4 * n1);
+        uint32_t* v1 = (uint32_t*)scratch(ctx, "lv1", 4 * n1);
         const int64_t nkeys = ncol * kNumBuckets;
         int64_t* hist = (int64_t*)scratch(ctx, "lhist", 8 * (size_t)(nkeys + 1));
         std::vector<int64_t> h(nkeys, 0);
         if (nv) {
-            VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv, d_colors, num_color,
-                      k0, (uint32_t*)lists.p);
+            VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv,
+                      ncol > 1 ? d_colors : nullptr, num_color, k0, (uint32_t*)lists.p);
             dzero(ctx, hist, 8 * nkeys);
             VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(nv, 256, 8192), 256, 0, (const uint32_t*)k0, nv, hist);
             bool alt = false;
@@ -596,24 +657,39 @@ struct Phase {
             if (alt) d2d(ctx, lists.p, v1, 4 * (size_t)nv);
             d2h(ctx, h.data(), hist, 8 * nkeys);
         }
-        sweeps.assign(ncol, {});
-        int64_t off = 0;
-        for (int64_t c = 0; c < ncol; c++)
+        sweeps.assign(ncol, ColorSweep{});
+        int64_t off = 0, max_hub = 0;
+        constexpr int kGroups[kFusedBuckets] = {256 / 4, 256 / 8, 256 / 16, 256 / 32, 256 / 64};
+        for (int64_t c = 0; c < ncol; c++) {
+            ColorSweep& cs = sweeps[c];
+            int64_t blocks = 0;
             for (int b = 0; b < kNumBuckets; b++) {
                 const int64_t cnt = h[c * kNumBuckets + b];
-                if (cnt) sweeps[c].push_back({off, cnt, b});
+                if (b < kFusedBuckets) {
+                    cs.plan.bbeg[b] = blocks;
+                    cs.plan.off[b] = off;
+                    cs.plan.n[b] = cnt;
+                    blocks += std::min<int64_t>((cnt + kGroups[b] - 1) / kGroups[b], 16384);
+                } else if (b == kHubBucket - 1) {
+                    cs.wide_off = off;
+                    cs.wide_n = cnt;
+                } else {
+                    cs.hub_off = off;
+                    cs.hub_n = cnt;
+                    max_hub = std::max(max_hub, cnt);
+                }
                 off += cnt;
             }
+            cs.plan.bbeg[kFusedBuckets] = blocks;
+            cs.fused_blocks = blocks;
+        }
         // Hub tables: one HBM table per resident wave, sized for the largest row.
         hub_cap = 0;
-        if (g->max_degree > kBucketMaxDeg[kHubBucket - 1]) {
+        if (max_hub > 0) {
             int64_t cap = 1;
             while (cap < 2 * (g->max_degree + 1)) cap <<= 1;
             hub_cap = cap;
-            int64_t nh = 0;
-            for (int64_t c = 0; c < ncol; c++)
-                for (auto& s : sweeps[c]) if (s.bucket == kHubBucket) nh = std::max(nh, s.n);
-            hub_blocks = (int)std::min<int64_t>(std::max<int64_t>(nh, 1), 1024);
+            hub_blocks = (int)std::min<int64_t>(max_hub, 1024);
             const size_t tn = (size_t)hub_cap * hub_blocks;
             int32_t* hk = (int32_t*)scratch(ctx, "hub_keys", 4 * tn);
             double* hv = (double*)scratch(ctx, "hub_vals", 8 * tn);
@@ -623,21 +699,21 @@ struct Phase {
         if (d_colors) {
             // colorIndex: vertices by color, ascending inside a color (louvain.cpp:662-677 run
             // by one thread).
-            corder = DBuf<int32_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
-            int64_t* ch = (int64_t*)scratch(ctx, "chist", 8 * (size_t)(ncol + 1));
-            std::vector<int64_t> hc(ncol, 0);
+            corder = DBuf<int32_t>(ctx, n1);
+            int64_t* ch = (int64_t*)scratch(ctx, "chist", 8 * (size_t)(num_color + 1));
+            std::vector<int64_t> hc(num_color, 0);
             if (nv) {
                 VT_LAUNCH(ctx, "color_keys", k_color_keys, grid_for(nv, 256, 8192), 256, 0, nv, d_colors, num_color, k0,
                           (uint32_t*)corder.p);
-                dzero(ctx, ch, 8 * ncol);
+                dzero(ctx, ch, 8 * num_color);
                 VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(nv, 256, 8192), 256, 0, (const uint32_t*)k0, nv, ch);
                 bool alt = false;
-                sort_pairs_u32(ctx, k0, (uint32_t*)corder.p, nv, (int)bits_for(ncol), k1, v1, &alt);
+                sort_pairs_u32(ctx, k0, (uint32_t*)corder.p, nv, (int)bits_for(num_color), k1, v1, &alt);
                 if (alt) d2d(ctx, corder.p, v1, 4 * (size_t)nv);
-                d2h(ctx, hc.data(), ch, 8 * ncol);
+                d2h(ctx, hc.data(), ch, 8 * num_color);
             }
-            color_ptr.assign(ncol + 1, 0);
-            for (int64_t c = 0; c < ncol; c++) color_ptr[c + 1] = color_ptr[c] + hc[c];
+            color_ptr.assign(num_color + 1, 0);
+            for (int64_t c = 0; c < num_color; c++) color_ptr[c + 1] = color_ptr[c] + hc[c];
         }
     }
 
@@ -659,34 +735,20 @@ struct Phase {
         return a;
     }
 
-    template <int G, int CAP, int BLOCK>
-    void launch_lds(const MoveArgs& a, const int32_t* list, int64_t n)
-    {
-        constexpr int NG = BLOCK / G;
-        const int64_t blocks = std::min<int64_t>((n + NG - 1) / NG, 2048 * 4);
-        auto kern = k_move_lds<G, CAP, BLOCK>;
-        VT_LAUNCH(ctx, "local_move", kern, (unsigned)blocks, BLOCK, 0, a, list, n);
-    }
-
-    void sweep_lists(const std::vector<SweepList>& sl)
+    void sweep(const ColorSweep& cs)
     {
         const MoveArgs a = move_args();
-        for (const auto& s : sl) {
-            const int32_t* list = lists.p + s.off;
-            switch (s.bucket) {
-                case 0: launch_lds<4, 16, 256>(a, list, s.n); break;
-                case 1: launch_lds<8, 32, 256>(a, list, s.n); break;
-                case 2: launch_lds<16, 64, 256>(a, list, s.n); break;
-                case 3: launch_lds<32, 128, 256>(a, list, s.n); break;
-                case 4: launch_lds<64, 256, 256>(a, list, s.n); break;
-                case 5: launch_lds<64, 2048, 64>(a, list, s.n); break;
-                default: {
-                    const int blocks = (int)std::min<int64_t>(s.n, hub_blocks);
-                    VT_LAUNCH(ctx, "local_move_hub", k_move_hbm, (unsigned)blocks, 64, 0, a, list, s.n,
-                              (int32_t*)scratch(ctx, "hub_keys", 0), (double*)scratch(ctx, "hub_vals", 0),
-                              (int32_t*)scratch(ctx, "hub_occ", 0), (uint32_t)hub_cap);
-                }
-            }
+        if (cs.fused_blocks)
+            VT_LAUNCH(ctx, "local_move", k_local_move, (unsigned)cs.fused_blocks, kFusedBlock, 0, a,
+                      (const int32_t*)lists.p, cs.plan);
+        if (cs.wide_n)
+            VT_LAUNCH(ctx, "local_move_wide", k_local_move_wide, (unsigned)std::min<int64_t>(cs.wide_n, 8192), 64, 0, a,
+                      (const int32_t*)(lists.p + cs.wide_off), cs.wide_n);
+        if (cs.hub_n) {
+            const int blocks = (int)std::min<int64_t>(cs.hub_n, hub_blocks);
+            VT_LAUNCH(ctx, "local_move_hub", k_move_hbm, (unsigned)blocks, 64, 0, a, (const int32_t*)(lists.p + cs.hub_off),
+                      cs.hub_n, (int32_t*)scratch(ctx, "hub_keys", 0), (double*)scratch(ctx, "hub_vals", 0),
+                      (int32_t*)scratch(ctx, "hub_occ", 0), (uint32_t)hub_cap);
         }
     }
 
@@ -730,7 +792,25 @@ struct Phase {
     }
 };
 
-double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out, int32_t* iters_out)
+}  // namespace vt
+
+// A phase in progress: device-resident state between iterations (the locals of
+// distLouvainMethod*, louvain.cpp:14-50).
+struct vt_phase {
+    vt::Phase ph;
+    vt_graph* g;
+    vt_phase_params p;
+    int mode = VT_MODE_PLAIN, et = 0;
+    double prevMod = 0.0, currMod = -1.0;
+    int numIters = 0;
+    bool done = false;
+    vt::DBuf<int64_t> dcolors;
+    vt_phase(vt_ctx* c, vt_graph* gr) : ph(c, gr), g(gr) {}
+};
+
+namespace vt {
+
+vt_phase* phase_begin(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p)
 {
     int mode = p->mode, et = p->et_type;
     if (mode < 0 || mode > 2) fail(ctx, VT_ERR_ARG, "vt_louvain_phase: bad mode %d", mode);
@@ -741,90 +821,118 @@ double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cv
         else if (!p->colors) fail(ctx, VT_ERR_ARG, "vt_louvain_phase: colors required for mode %d", mode);
     }
     const int64_t nv = g->nv;
-    Phase ph(ctx, g);
-    ph.alloc(et != 0);
-    ph.init(p->flags);
-
-    DBuf<int64_t> dcolors;
-    if (mode != VT_MODE_PLAIN) {
-        std::vector<int64_t> hc(p->colors, p->colors + nv);
-        for (int64_t i = 0; i < nv; i++)
-            if (hc[i] > p->num_color - 1 || hc[i] < -1)
-                fail(ctx, VT_ERR_ARG, "vt_louvain_phase: color %lld of vertex %lld outside [-1, num_color-1]",
-                     (long long)hc[i], (long long)i);
-        dcolors = DBuf<int64_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
-        h2d(ctx, dcolors.p, hc.data(), 8 * (size_t)nv);
+    vt_phase* st = new vt_phase(ctx, g);
+    try {
+        st->p = *p;
+        st->p.colors = nullptr;
+        st->mode = mode;
+        st->et = et;
+        st->prevMod = p->lower;
+        Phase& ph = st->ph;
+        ph.alloc(et != 0);
+        ph.init(p->flags);
+        if (mode != VT_MODE_PLAIN) {
+            for (int64_t i = 0; i < nv; i++)
+                if (p->colors[i] > p->num_color - 1 || p->colors[i] < -1)
+                    fail(ctx, VT_ERR_ARG, "vt_louvain_phase: color %lld of vertex %lld outside [-1, num_color-1]",
+                         (long long)p->colors[i], (long long)i);
+            st->dcolors = DBuf<int64_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
+            h2d(ctx, st->dcolors.p, p->colors, 8 * (size_t)nv);
+        }
+        // Sweep lists per color for COLOR; one Jacobi sweep (event order = colorIndex) for VORDER.
+        ph.build_lists(mode != VT_MODE_PLAIN ? st->dcolors.p : nullptr, p->num_color, mode == VT_MODE_VORDER);
+    } catch (...) {
+        delete st;
+        throw;
     }
-    // Sweep lists: per color for COLOR; a single Jacobi sweep for PLAIN and VORDER.
-    ph.build_lists(mode == VT_MODE_COLOR ? dcolors.p : nullptr, mode == VT_MODE_COLOR ? p->num_color : 1);
-    if (mode == VT_MODE_VORDER) {
-        // event order = colorIndex (color-major, vertex order inside)
-        Phase tmp(ctx, g);
-        tmp.build_lists(dcolors.p, p->num_color);
-        ph.corder = std::move(tmp.corder);
-        ph.color_ptr = tmp.color_ptr;
-    }
-
     ctx->trace_q.clear();
     ctx->trace_moves.clear();
-    double prevMod = p->lower, currMod = -1.0;
-    int numIters = 0;
-    const double one_minus_delta = 1.0 - p->et_delta;
-    for (;;) {
-        numIters++;
-        if (ph.et) dzero(ctx, ph.ucount.p, 8);
-        int64_t moves = 0;
-        if (mode == VT_MODE_COLOR) {
-            for (int64_t ci = 0; ci < p->num_color; ci++) {
-                ph.sweep_lists(ph.sweeps[ci]);
-                const int64_t a0 = ph.color_ptr[ci], a1 = ph.color_ptr[ci + 1];
-                moves += ph.apply_moves(ph.corder.p + a0, a1 - a0, ph.cdeg.p, ph.csize.p);
-            }
-        } else {
-            ph.sweep_lists(ph.sweeps[0]);
-        }
-        if (et == 3 || et == 4) {
-            unsigned long long vc = 0;
-            d2h(ctx, &vc, ph.ucount.p, 8);
-            if (vc >= 90) {  // ET_CUTOFF compared with a count (louvain.cpp:116-121)
-                currMod = -1.0;
-                ctx->trace_q.push_back(NAN);
-                ctx->trace_moves.push_back(moves);
-                break;
-            }
-        }
-        if (mode == VT_MODE_PLAIN) moves = ph.apply_moves(nullptr, nv, ph.cdeg.p, ph.csize.p);
-        else if (mode == VT_MODE_VORDER) moves = ph.apply_moves(ph.corder.p, nv, ph.cdeg.p, ph.csize.p);
+    return st;
+}
 
-        currMod = ph.modularity(p->flags);
-        ctx->trace_q.push_back(currMod);
-        ctx->trace_moves.push_back(moves);
-        if ((currMod - prevMod) < p->thresh) break;
-        prevMod = currMod;
-        if (prevMod < p->lower) prevMod = p->lower;
-        if (p->max_iters > 0 && numIters >= p->max_iters) break;
-
-        if (et == 0) {
-            // past <- curr <- target <- old past (louvain.cpp:565-570), by pointer rotation.
-            std::swap(ph.past.p, ph.curr.p);
-            std::swap(ph.curr.p, ph.target.p);
-        } else if (et == 1 || et == 3) {
-            VT_LAUNCH(ctx, "rotate_et", k_rotate_et1, grid_for(nv, 256, 8192), 256, 0, nv, numIters, ph.past.p,
-                      ph.curr.p, ph.target.p, ph.active.p);
-        } else {
-            VT_LAUNCH(ctx, "rotate_et", k_rotate_et2, grid_for(nv, 256, 8192), 256, 0, nv, numIters, one_minus_delta,
-                      ph.past.p, ph.curr.p, ph.target.p, ph.active.p, ph.pa.p, (const double*)ph.pb.p);
-            std::swap(ph.pa.p, ph.pb.p);  // std::swap(p_curr, p_prev)
+// One iteration of the reference loop body (louvain.cpp:473-577 and siblings); sets done
+// when the reference would leave the loop.
+void phase_step(vt_ctx* ctx, vt_phase* st)
+{
+    if (st->done) return;
+    Phase& ph = st->ph;
+    const vt_phase_params& p = st->p;
+    const int64_t nv = ph.nv;
+    st->numIters++;
+    if (ph.et) dzero(ctx, ph.ucount.p, 8);
+    int64_t moves = 0;
+    if (st->mode == VT_MODE_COLOR) {
+        for (int64_t ci = 0; ci < p.num_color; ci++) {
+            ph.sweep(ph.sweeps[ci]);
+            const int64_t a0 = ph.color_ptr[ci], a1 = ph.color_ptr[ci + 1];
+            moves += ph.apply_moves(ph.corder.p + a0, a1 - a0, ph.cdeg.p, ph.csize.p);
+        }
+    } else {
+        ph.sweep(ph.sweeps[0]);
+    }
+    if (st->et == 3 || st->et == 4) {
+        unsigned long long vc = 0;
+        d2h(ctx, &vc, ph.ucount.p, 8);
+        if (vc >= 90) {  // ET_CUTOFF compared with a count (louvain.cpp:116-121)
+            st->currMod = -1.0;
+            ctx->trace_q.push_back(NAN);
+            ctx->trace_moves.push_back(moves);
+            st->done = true;
+            return;
         }
     }
+    if (st->mode == VT_MODE_PLAIN) moves = ph.apply_moves(nullptr, nv, ph.cdeg.p, ph.csize.p);
+    else if (st->mode == VT_MODE_VORDER) moves = ph.apply_moves(ph.corder.p, nv, ph.cdeg.p, ph.csize.p);
 
-    // cvect = pastComm (louvain.cpp:579); keep a device copy for the rebuild.
+    st->currMod = ph.modularity(p.flags);
+    ctx->trace_q.push_back(st->currMod);
+    ctx->trace_moves.push_back(moves);
+    if ((st->currMod - st->prevMod) < p.thresh) { st->done = true; return; }
+    st->prevMod = st->currMod;
+    if (st->prevMod < p.lower) st->prevMod = p.lower;
+    if (p.max_iters > 0 && st->numIters >= p.max_iters) { st->done = true; return; }
+
+    if (st->et == 0) {
+        // past <- curr <- target <- old past (louvain.cpp:565-570), by pointer rotation.
+        std::swap(ph.past.p, ph.curr.p);
+        std::swap(ph.curr.p, ph.target.p);
+    } else if (st->et == 1 || st->et == 3) {
+        VT_LAUNCH(ctx, "rotate_et", k_rotate_et1, grid_for(nv, 256, 8192), 256, 0, nv, st->numIters, ph.past.p,
+                  ph.curr.p, ph.target.p, ph.active.p);
+    } else {
+        VT_LAUNCH(ctx, "rotate_et", k_rotate_et2, grid_for(nv, 256, 8192), 256, 0, nv, st->numIters,
+                  1.0 - p.et_delta, ph.past.p, ph.curr.p, ph.target.p, ph.active.p, ph.pa.p, (const double*)ph.pb.p);
+        std::swap(ph.pa.p, ph.pb.p);  // std::swap(p_curr, p_prev)
+    }
+}
+
+// cvect = pastComm, iters = numIters, return prevMod (louvain.cpp:579-590); a device copy
+// of the partition stays on the graph for the rebuild.
+double phase_end(vt_ctx* ctx, vt_phase* st, int64_t* cvect_out, int32_t* iters_out)
+{
+    Phase& ph = st->ph;
+    vt_graph* g = st->g;
+    const int64_t nv = g->nv;
     if (!g->part) g->part = (int64_t*)dalloc(ctx, 8 * (size_t)std::max<int64_t>(nv, 1));
     d2d(ctx, g->part, ph.past.p, 8 * (size_t)nv);
     if (cvect_out) d2h(ctx, cvect_out, ph.past.p, 8 * (size_t)nv);
     sync(ctx);
-    *iters_out = numIters;
-    return prevMod;
+    if (iters_out) *iters_out = st->numIters;
+    return st->prevMod;
+}
+
+double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out, int32_t* iters_out)
+{
+    vt_phase* st = phase_begin(ctx, g, p);
+    try {
+        while (!st->done) phase_step(ctx, st);
+        const double q = phase_end(ctx, st, cvect_out, iters_out);
+        delete st;
+        return q;
+    } catch (...) {
+        delete st;
+        throw;
+    }
 }
 
 }  // namespace vt
@@ -844,6 +952,69 @@ int vt_louvain_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t
         if (iters_out) *iters_out = it;
         if (q_out) *q_out = q;
     });
+}
+
+int vt_phase_begin(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, vt_phase** out)
+{
+    if (!ctx || !g || !p || !out) return VT_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        *out = phase_begin(ctx, g, p);
+        sync(ctx);
+    });
+}
+
+int vt_phase_step(vt_ctx* ctx, vt_phase* st, double* q_out, int32_t* done_out)
+{
+    if (!ctx || !st) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        phase_step(ctx, st);
+        sync(ctx);
+        if (q_out) *q_out = st->currMod;
+        if (done_out) *done_out = st->done ? 1 : 0;
+    });
+}
+
+int vt_phase_sweep(vt_ctx* ctx, vt_phase* st)
+{
+    if (!ctx || !st) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        for (const auto& cs : st->ph.sweeps) st->ph.sweep(cs);
+        sync(ctx);
+    });
+}
+
+int vt_phase_get_state(vt_ctx* ctx, const vt_phase* st, int64_t* curr, int64_t* csize, double* cdeg, double* vdeg,
+                       double* constant, int64_t* target, double* cw)
+{
+    if (!ctx || !st) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        const Phase& ph = st->ph;
+        const size_t b8 = 8 * (size_t)ph.nv;
+        if (curr) d2h(ctx, curr, ph.curr.p, b8);
+        if (csize) d2h(ctx, csize, ph.csize.p, b8);
+        if (cdeg) d2h(ctx, cdeg, ph.cdeg.p, b8);
+        if (vdeg) d2h(ctx, vdeg, ph.vdeg.p, b8);
+        if (target) d2h(ctx, target, ph.target.p, b8);
+        if (cw) d2h(ctx, cw, ph.cw.p, b8);
+        if (constant) d2h(ctx, constant, ph.scal.p + 2, 8);
+    });
+}
+
+int vt_phase_end(vt_ctx* ctx, vt_phase* st, int64_t* cvect_out, int32_t* iters_out, double* q_out)
+{
+    if (!ctx || !st) return VT_ERR_ARG;
+    const int rc = guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        const double q = phase_end(ctx, st, cvect_out, iters_out);
+        if (q_out) *q_out = q;
+    });
+    delete st;
+    return rc;
 }
 
 int vt_last_phase_trace(const vt_ctx* ctx, int32_t cap, double* q, int64_t* moves, int32_t* n)
@@ -890,8 +1061,8 @@ int vt_sweep(vt_ctx* ctx, const vt_graph* g, const int64_t* curr, const int64_t*
         h2d(ctx, ph.cdeg.p, cdeg, b8);
         h2d(ctx, ph.vdeg.p, vdeg, b8);
         h2d(ctx, ph.constant(), &constant, 8);
-        ph.build_lists(nullptr, 1);
-        ph.sweep_lists(ph.sweeps[0]);
+        ph.build_lists(nullptr, 1, false);
+        ph.sweep(ph.sweeps[0]);
         if (target_out) d2h(ctx, target_out, ph.target.p, b8);
         if (cw_out) d2h(ctx, cw_out, ph.cw.p, b8);
         if (dsize_out || ddeg_out) {
