@@ -75,7 +75,21 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttq", action="store_true", help="skip the time-to-final-Q run")
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
+
+    if args.probe_copy:
+        import torch
+        n = args.probe_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "copy", "bytes_read_per_launch": 8 * n, "bytes_written_per_launch": 8 * n,
+                          "launches": 5}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -163,8 +163,11 @@ int main(int argc, char* argv[])
 
 #else  // shared-library replay API
 
+// Every replay entry point runs the reference on one OpenMP thread: its reductions for 2m and
+// modularity are ordered by the thread count, and the oracle restates the one-thread order.
 static void ensure_mpi()
 {
+    omp_set_num_threads(1);
     int inited = 0;
     MPI_Initialized(&inited);
     if (!inited) {

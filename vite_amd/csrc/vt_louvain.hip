@@ -19,6 +19,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -47,7 +48,10 @@ __device__ __forceinline__ uint32_t slot_hash(int32_t c) { return (uint32_t)c * 
 
 // Degree buckets: (lanes per vertex G, table capacity CAP, max degree).  CAP >= 2*(deg+1)
 // keeps the table at most half full (a row has at most deg+1 distinct communities).
-constexpr int kNumBuckets = 7;
+constexpr int kNumBuckets = 8;
+constexpr int kLpvBucket = 7;      // lane per vertex (degree <= lpv_max)
+constexpr int kLpvMaxDeg = 64;     // the largest row k_local_move_lpv is instantiated for
+constexpr int kLpvArena = 2560;
 constexpr int kHubBucket = 6;
 
 __host__ __device__ inline int degree_bucket(int64_t d)
@@ -284,10 +288,281 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
     wave_sync();
 }
 
+// Insert-or-find without an occupancy list: CAS first (one LDS round trip whether the key
+// is new or already present).
+__device__ __forceinline__ int32_t lds_slot(int32_t* keys, uint32_t capmask, int32_t c)
+{
+    uint32_t h = (slot_hash(c) >> 7) & capmask;
+    for (;;) {
+        const int32_t old = atomicCAS(&keys[h], -1, c);
+        if (old == -1 || old == c) return (int32_t)h;
+        h = (h + 1) & capmask;
+    }
+}
+
+// Folds the weights of the lanes in `rest` (ascending lane = edge order) onto acc, reading
+// the staged weights eight at a time so the LDS latency is paid once per eight terms.
+template <class mask_t>
+__device__ __forceinline__ double fold_rest(double acc, mask_t rest, const double* wst)
+{
+    while (rest) {
+        int k[8];
+        bool ok[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            ok[u] = rest != 0;
+            k[u] = ok[u] ? (int)(sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)rest)
+                                                     : __builtin_ctz((unsigned)rest))
+                         : 0;
+            rest = ok[u] ? (mask_t)(rest & (rest - 1)) : rest;
+        }
+        double w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) w[u] = wst[k[u]];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (ok[u]) acc = acc + w[u];
+    }
+    return acc;
+}
+
+// One chunk of at most G edges of a row already in registers: insert / find each
+// community, elect the lowest lane per community, and let it fold the chunk's weights for
+// that community in edge order onto the running counter.  A counter created here starts as
+// the first weight, or 0.0 + w for the vertex's own community, whose counter the reference
+// seeds with 0.0 (louvain.cpp:2326-2327, 2456-2462).
+template <int G>
+__device__ __forceinline__ void fold_chunk(const Table<G, false>& t, double* ecc_cell, int32_t c, int32_t cc, double w,
+                                           bool valid, bool self, int lane, int gbase, double& selfLoop)
+{
+    using mask_t = typename MaskT<G>::type;
+    const mask_t gmask = (G == 64) ? (mask_t)~0ull : (mask_t)((1ull << G) - 1);
+    const int32_t slot = valid ? lds_slot(t.keys, t.capmask, c) : -1;
+    t.wst[lane] = w;
+    if (valid) atomicOr(&t.masks[slot], (mask_t)1 << lane);
+    wave_sync();
+    const mask_t m = valid ? t.masks[slot] : (mask_t)0;
+    wave_sync();
+    if (valid && (m & (((mask_t)1 << lane) - 1)) == 0) {
+        double acc = t.vals[slot];
+        acc = is_sent(acc) ? (c == cc ? 0.0 + w : w) : acc + w;
+        acc = fold_rest<mask_t>(acc, (mask_t)(m & (m - 1)), t.wst);
+        t.vals[slot] = acc;
+        t.masks[slot] = 0;
+        if (c == cc) *ecc_cell = acc;
+    }
+    mask_t sm = (mask_t)((__ballot(self) >> gbase) & gmask);
+    while (sm) {
+        const int k = (G == 64) ? __builtin_ctzll((unsigned long long)sm) : __builtin_ctz((unsigned)sm);
+        sm &= sm - 1;
+        selfLoop = selfLoop + t.wst[k];
+    }
+    wave_sync();
+}
+
+// Per-vertex header, prefetched one vertex ahead.
+// A vertex as the fused buckets see it: row start, degree (< 0 for a vertex early
+// termination froze), own community and weighted degree.  Rows here have degree <= 127 and
+// community ids fit 32 bits (vt_graph_upload rejects nv >= 2^31).
+struct VHeader {
+    int32_t v;
+    int32_t deg;
+    int64_t e0;
+    int32_t cc;
+    double kv;
+};
+
+__device__ __forceinline__ VHeader load_header(const MoveArgs& a, int32_t v, bool have)
+{
+    VHeader h;
+    h.v = v;
+    h.deg = -1;
+    h.e0 = 0;
+    h.cc = 0;
+    h.kv = 0.0;
+    if (have) {
+        const bool live = !(a.active && !a.active[v]);
+        h.e0 = a.offs[v];
+        h.deg = live ? (int32_t)(a.offs[v + 1] - h.e0) : -1;
+        h.cc = (int32_t)a.curr[v];
+        h.kv = a.vdeg[v];
+    }
+    return h;
+}
+
+#ifndef VT_PIPE_ROWS
+#define VT_PIPE_ROWS 0
+#endif
+constexpr bool kPipeRows = VT_PIPE_ROWS;
+
+// Rows of degree <= 2G - 1 (every fused bucket), table CAP = 4G slots, software-pipelined
+// across the vertices a group processes: while vertex i is folded in LDS and its candidates'
+// community degrees are gathered, vertex i+1's two edge chunks and neighbour-community
+// gathers are already in flight.  Same results as move_vertex.
+
+struct RowRegs {
+    int32_t t0, t1;   // neighbour ids (low words; ids < 2^31)
+    double w0, w1;
+    int32_t c0, c1;   // their communities
+    bool ok0, ok1, self0, self1;
+};
+
+// Edge records are {int64 tail, f64 weight}; only the tail's low word and the weight are
+// read, which keeps a chunk in flight at three registers a lane.
+template <int G>
+__device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader& h, int lane, RowRegs& r)
+{
+    const int64_t j0 = h.e0 + lane, j1 = h.e0 + G + lane;
+    const int32_t* words = reinterpret_cast<const int32_t*>(a.edges);
+    const double* wts = reinterpret_cast<const double*>(a.edges);
+    r.ok0 = lane < h.deg;
+    r.ok1 = G + lane < h.deg;
+    r.t0 = r.t1 = -1;
+    r.w0 = r.w1 = 0.0;
+    if (r.ok0) {
+        r.t0 = words[4 * j0];
+        r.w0 = wts[2 * j0 + 1];
+    }
+    if (r.ok1) {
+        r.t1 = words[4 * j1];
+        r.w1 = wts[2 * j1 + 1];
+    }
+}
+
+__device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, RowRegs& r)
+{
+    const int32_t* cw = reinterpret_cast<const int32_t*>(a.curr);
+    r.c0 = 0;
+    r.c1 = 0;
+    if (r.ok0) r.c0 = cw[2 * (int64_t)r.t0];
+    if (r.ok1) r.c1 = cw[2 * (int64_t)r.t1];
+    r.self0 = r.ok0 && r.t0 == v;
+    r.self1 = r.ok1 && r.t1 == v;
+}
+
+template <int G, int CAP, int BLOCK, bool PIPE>
+__device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Table<G, false>& t, double* ecc_cell,
+                                                  const int32_t* __restrict__ list, int64_t n, int64_t idx,
+                                                  int64_t stride, int lane, int gbase, unsigned* inactive_local)
+{
+    static_assert(CAP == 4 * G, "four slots per lane");
+    if (idx >= n) return;
+    VHeader h = load_header(a, list[idx], true);
+    int32_t vn = (idx + stride < n) ? list[idx + stride] : 0;
+    RowRegs r;
+    if constexpr (PIPE) row_issue_edges<G>(a, h, lane, r);
+    VHeader hn = load_header(a, vn, idx + stride < n);
+    if constexpr (PIPE) row_issue_gathers(a, h.v, r);
+    const double constant = *a.constant;
+    while (idx < n) {
+        const bool has_next = PIPE && idx + stride < n;
+        // list entry two ahead, header one ahead (and with PIPE the next vertex's edges)
+        const int64_t i2 = idx + 2 * stride;
+        const int32_t vnn = (i2 < n) ? list[i2] : 0;
+        RowRegs rn;
+        if constexpr (PIPE) {
+            row_issue_edges<G>(a, hn, lane, rn);
+        } else {
+            row_issue_edges<G>(a, h, lane, r);
+            row_issue_gathers(a, h.v, r);
+        }
+        const VHeader hnn = load_header(a, vnn, i2 < n);
+
+        const int32_t v = h.v;
+        const int64_t cc = h.cc;
+        if (h.deg < 0) {
+            if (lane == 0) {
+                a.cw[v] = a.frozen[v];
+                atomicAdd(inactive_local, 1u);
+            }
+        } else if (h.deg == 0) {
+            if (lane == 0) {
+                a.target[v] = cc;
+                a.cw[v] = 0.0;
+                if (a.frozen) a.frozen[v] = 0.0;
+            }
+        } else {
+            const double ccDeg = a.cdeg[cc];
+            const int64_t ccSize = a.csize[cc];
+            if (lane == 0) *ecc_cell = 0.0;
+            double selfLoop = 0.0;
+            fold_chunk<G>(t, ecc_cell, r.c0, (int32_t)cc, r.w0, r.ok0, r.self0, lane,
+                          gbase, selfLoop);
+            if (h.deg > G)
+                fold_chunk<G>(t, ecc_cell, r.c1, (int32_t)cc, r.w1, r.ok1, r.self1,
+                              lane, gbase, selfLoop);
+            // scan this lane's four slots, clearing them as they are read
+            int32_t ck[4];
+            double ey[4], ay[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int sl = lane + u * G;
+                ck[u] = t.keys[sl];
+                ey[u] = t.vals[sl];
+            }
+            const double ecc = *ecc_cell;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int sl = lane + u * G;
+                t.keys[sl] = -1;
+                t.vals[sl] = sent_value();
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                ok[u] = ck[u] != -1 && (int64_t)ck[u] != cc;
+                ay[u] = ok[u] ? a.cdeg[ck[u]] : 0.0;
+            }
+            // the next vertex's neighbour communities, before waiting on this one's gathers
+            if (has_next) row_issue_gathers(a, hn.v, rn);
+            // distGetMaxIndex (louvain.cpp:2246-2276)
+            const double kv = h.kv;
+            const double eix = ecc - selfLoop;
+            const double ax = ccDeg - kv;
+            double bg = 0.0;
+            long long bc = LLONG_MAX;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (ok[u]) {
+                    const double gain = 2.0 * (ey[u] - eix) - 2.0 * kv * (ay[u] - ax) * constant;
+                    if (gain > 0.0 && (gain > bg || (gain == bg && (long long)ck[u] < bc))) {
+                        bg = gain;
+                        bc = ck[u];
+                    }
+                }
+            }
+            for (int o = G / 2; o > 0; o >>= 1) {
+                const double og = __shfl_xor(bg, o, G);
+                const long long oc = __shfl_xor(bc, o, G);
+                if (og > bg || (og == bg && oc < bc)) {
+                    bg = og;
+                    bc = oc;
+                }
+            }
+            if (lane == 0) {
+                int64_t tgt = cc;
+                if (bg > 0.0) {
+                    tgt = bc;
+                    if (ccSize == 1 && tgt > cc && a.csize[tgt] == 1) tgt = cc;  // louvain.cpp:2275-2276
+                }
+                a.target[v] = tgt;
+                a.cw[v] = ecc;
+                if (a.frozen) a.frozen[v] = ecc;
+            }
+            wave_sync();
+        }
+        if (has_next && h.deg <= 0) row_issue_gathers(a, hn.v, rn);
+        h = hn;
+        hn = hnn;
+        if constexpr (PIPE) r = rn;
+        idx += stride;
+    }
+}
+
 // LDS carve-out for BLOCK/G groups of CAP-slot tables.
 constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-template <int G, int CAP, int BLOCK>
+template <int G, int CAP, int BLOCK, bool OCC = true>
 struct LdsLayout {
     using mask_t = typename MaskT<G>::type;
     static constexpr int NG = BLOCK / G;
@@ -295,19 +570,20 @@ struct LdsLayout {
     static constexpr size_t vals = align16(keys + sizeof(int32_t) * NG * CAP);
     static constexpr size_t masks = align16(vals + sizeof(double) * NG * CAP);
     static constexpr size_t occ = align16(masks + sizeof(mask_t) * NG * CAP);
-    static constexpr size_t wst = align16(occ + sizeof(int32_t) * NG * CAP);
+    static constexpr size_t wst = align16(occ + (OCC ? sizeof(int32_t) * NG * CAP : 0));
     static constexpr size_t nocc = align16(wst + sizeof(double) * NG * G);
     static constexpr size_t cslot = align16(nocc + sizeof(int32_t) * NG);
-    static constexpr size_t bytes = align16(cslot + sizeof(int32_t) * NG);
+    static constexpr size_t ecc = align16(cslot + sizeof(int32_t) * NG);
+    static constexpr size_t bytes = align16(ecc + sizeof(double) * NG);
 };
 
 // Runs one degree bucket inside a block: block `blk` of `nblk` owns list entries
 // blk*NG + g, stepping by nblk*NG.  Every thread of the block must call it.
-template <int G, int CAP, int BLOCK>
+template <int G, int CAP, int BLOCK, bool TWO>
 __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const int32_t* __restrict__ list, int64_t n,
                                            int64_t blk, int64_t nblk, unsigned* inactive)
 {
-    using L = LdsLayout<G, CAP, BLOCK>;
+    using L = LdsLayout<G, CAP, BLOCK, !TWO>;
     using mask_t = typename L::mask_t;
     constexpr int NG = L::NG;
     int32_t* keys = (int32_t*)(lds + L::keys);
@@ -329,16 +605,23 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
     const int gbase = (threadIdx.x % 64) / G * G;
     Table<G, false> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, occ + g * CAP, wst + g * G, nullptr,
                       nocc + g, cslot + g, (uint32_t)(CAP - 1)};
-    for (int64_t idx = blk * NG + g; idx < n; idx += nblk * NG) move_vertex<G, false>(a, t, list[idx], lane, gbase, inactive);
+    const int64_t stride = nblk * NG;
+    int64_t idx = blk * NG + g;
+    if constexpr (TWO) {
+        double* ecc_cell = (double*)(lds + L::ecc) + g;
+        run_two_pipelined<G, CAP, BLOCK, kPipeRows>(a, t, ecc_cell, list, n, idx, stride, lane, gbase, inactive);
+    } else {
+        for (; idx < n; idx += stride) move_vertex<G, false>(a, t, list[idx], lane, gbase, inactive);
+    }
 }
 
 // Buckets 0-4 (degree <= 127) of one sweep in a single launch: blocks [bbeg[b], bbeg[b+1])
 // run bucket b over its list range.
 constexpr int kFusedBuckets = 5;
 constexpr int kFusedBlock = 256;
-constexpr size_t kFusedLds = std::max({LdsLayout<4, 16, 256>::bytes, LdsLayout<8, 32, 256>::bytes,
-                                       LdsLayout<16, 64, 256>::bytes, LdsLayout<32, 128, 256>::bytes,
-                                       LdsLayout<64, 256, 256>::bytes});
+constexpr size_t kFusedLds = std::max({LdsLayout<4, 16, 256, false>::bytes, LdsLayout<8, 32, 256, false>::bytes,
+                                       LdsLayout<16, 64, 256, false>::bytes, LdsLayout<32, 128, 256, false>::bytes,
+                                       LdsLayout<64, 256, 256, false>::bytes});
 
 struct SweepPlan {
     int64_t bbeg[kFusedBuckets + 1];  // block ranges
@@ -346,7 +629,11 @@ struct SweepPlan {
     int64_t n[kFusedBuckets];         // list lengths
 };
 
-__global__ void __launch_bounds__(kFusedBlock) k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
+#ifndef VT_FUSED_WPE
+#define VT_FUSED_WPE 1
+#endif
+__global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(VT_FUSED_WPE)))
+k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
 {
     __shared__ __align__(16) char lds[kFusedLds];
     __shared__ unsigned s_inactive;
@@ -356,11 +643,11 @@ __global__ void __launch_bounds__(kFusedBlock) k_local_move(MoveArgs a, const in
     const int64_t blk = blockIdx.x - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
     const int32_t* list = lists + plan.off[b];
     switch (b) {
-        case 0: run_bucket<4, 16, 256>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
-        case 1: run_bucket<8, 32, 256>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
-        case 2: run_bucket<16, 64, 256>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
-        case 3: run_bucket<32, 128, 256>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
-        default: run_bucket<64, 256, 256>(a, lds, list, plan.n[4], blk, nblk, &s_inactive); break;
+        case 0: run_bucket<4, 16, 256, true>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
+        case 1: run_bucket<8, 32, 256, true>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+        case 2: run_bucket<16, 64, 256, true>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
+        case 3: run_bucket<32, 128, 256, true>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+        default: run_bucket<64, 256, 256, true>(a, lds, list, plan.n[4], blk, nblk, &s_inactive); break;
     }
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
@@ -372,9 +659,167 @@ __global__ void __launch_bounds__(64) k_local_move_wide(MoveArgs a, const int32_
     __shared__ __align__(16) char lds[LdsLayout<64, 2048, 64>::bytes];
     __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
-    run_bucket<64, 2048, 64>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
+    run_bucket<64, 2048, 64, false>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
+}
+
+// ---- lane per vertex ---------------------------------------------------------------------
+//
+// For rows of degree <= MAXD one lane owns one vertex and walks its row in CSR order, so the
+// per-community counters are the reference's left folds with no cross-lane traffic.  The
+// lane's map (louvain.cpp:2297-2298) is a compact key/value array in LDS holding at most
+// deg+1 communities, searched linearly; the wave carves lane regions out of an ARENA-entry
+// LDS pool by a prefix sum of deg+1 and runs extra rounds for lanes that did not fit.  Row
+// loads run three stages deep: edges two batches ahead, the community gather one batch
+// ahead of the map updates.  Blocks map to list ranges so that each XCD (blocks are dealt
+// to the eight XCDs round robin) sweeps one contiguous range and keeps its gathers in its L2.
+
+__device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t nb)
+{
+    const int64_t x = b % 8, i = b / 8, per = nb / 8, rem = nb % 8;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+template <int MAXD, int ARENA, int U>
+__global__ void __launch_bounds__(64) k_local_move_lpv(MoveArgs a, const int32_t* __restrict__ list, int64_t n,
+                                                       int64_t chunk)
+{
+    __shared__ int32_t s_keys[ARENA];
+    __shared__ double s_vals[ARENA];
+    const int lane = threadIdx.x;
+    const int64_t lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int64_t lo = lb * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    const double constant = *a.constant;
+    unsigned inactive = 0;
+
+    for (int64_t base = lo; base < hi; base += 64) {
+        const int64_t idx = base + lane;
+        bool pending = false;
+        int32_t v = 0;
+        int64_t e0 = 0, e1 = 0, cc = 0;
+        if (idx < hi) {
+            v = list[idx];
+            if (a.active && !a.active[v]) {
+                a.cw[v] = a.frozen[v];  // louvain.cpp:101-104
+                inactive++;
+            } else {
+                e0 = a.offs[v];
+                e1 = a.offs[v + 1];
+                cc = a.curr[v];
+                if (e0 == e1) {  // isolated: stays, clusterWeight 0 (louvain.cpp:2323-2335)
+                    a.target[v] = cc;
+                    a.cw[v] = 0.0;
+                    if (a.frozen) a.frozen[v] = 0.0;
+                } else {
+                    pending = true;
+                }
+            }
+        }
+        while (__ballot(pending)) {
+            const int need = pending ? (int)(e1 - e0 + 1) : 0;
+            int incl = need;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            if (pending && incl <= ARENA) {
+                pending = false;
+                int32_t* keys = s_keys + (incl - need);
+                double* vals = s_vals + (incl - need);
+                const double kv = a.vdeg[v];
+                const double ccDeg = a.cdeg[cc];
+                const int64_t ccSize = a.csize[cc];
+                keys[0] = (int32_t)cc;  // own community first, counter 0.0 (louvain.cpp:2326-2327)
+                vals[0] = 0.0;
+                int nk = 1;
+                double selfLoop = 0.0;
+
+                DEdge eA[U], eB[U], eC[U];
+                int32_t cA[U], cB[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (e0 + u < e1) eA[u] = a.edges[e0 + u];
+                    if (e0 + U + u < e1) eB[u] = a.edges[e0 + U + u];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (e0 + u < e1) cA[u] = (int32_t)a.curr[eA[u].tail];
+                for (int64_t j = e0; j < e1; j += U) {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (j + 2 * U + u < e1) eC[u] = a.edges[j + 2 * U + u];
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (j + U + u < e1) cB[u] = (int32_t)a.curr[eB[u].tail];
+                    // distBuildLocalMapCounter (louvain.cpp:2431-2463) for this batch, in order
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        if (j + u < e1) {
+                            const double w = eA[u].weight;
+                            if (eA[u].tail == (int64_t)v) selfLoop = selfLoop + w;
+                            const int32_t c = cA[u];
+                            int k = 0;
+                            while (k < nk && keys[k] != c) k++;
+                            if (k < nk) {
+                                vals[k] = vals[k] + w;
+                            } else {
+                                keys[nk] = c;
+                                vals[nk] = w;
+                                nk++;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        eA[u] = eB[u];
+                        cA[u] = cB[u];
+                        eB[u] = eC[u];
+                    }
+                }
+
+                // distGetMaxIndex (louvain.cpp:2246-2276): gathers in batches of 2U.
+                const double ecc = vals[0];
+                const double eix = ecc - selfLoop;
+                const double ax = ccDeg - kv;
+                double bg = 0.0;
+                int32_t bc = -1;
+                for (int k0 = 1; k0 < nk; k0 += 2 * U) {
+                    int32_t ck[2 * U];
+                    double ay[2 * U];
+#pragma unroll
+                    for (int u = 0; u < 2 * U; u++)
+                        if (k0 + u < nk) ck[u] = keys[k0 + u];
+#pragma unroll
+                    for (int u = 0; u < 2 * U; u++)
+                        if (k0 + u < nk) ay[u] = a.cdeg[ck[u]];
+#pragma unroll
+                    for (int u = 0; u < 2 * U; u++) {
+                        if (k0 + u < nk) {
+                            const double gain = 2.0 * (vals[k0 + u] - eix) - 2.0 * kv * (ay[u] - ax) * constant;
+                            if (gain > 0.0 && (gain > bg || (gain == bg && ck[u] < bc))) {
+                                bg = gain;
+                                bc = ck[u];
+                            }
+                        }
+                    }
+                }
+                int64_t tgt = cc;
+                if (bg > 0.0) {
+                    tgt = bc;
+                    // singleton rule (louvain.cpp:2275-2276)
+                    if (ccSize == 1 && tgt > cc && a.csize[tgt] == 1) tgt = cc;
+                }
+                a.target[v] = tgt;
+                a.cw[v] = ecc;
+                if (a.frozen) a.frozen[v] = ecc;
+            }
+            wave_sync();  // arena reuse in the next round
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) inactive += __shfl_xor(inactive, o);
+    if (lane == 0 && inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)inactive);
 }
 
 // Hub kernel: one wave per vertex, table in HBM (capacity `cap` per wave slot).
@@ -442,10 +887,11 @@ __global__ void k_constant(const double* __restrict__ sums, double* __restrict__
 
 // Sort keys for the sweep lists: (color bucket, degree bucket).
 __global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const int64_t* __restrict__ colors,
-                            int64_t num_color, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                            int64_t num_color, int lpv_max, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = (uint32_t)degree_bucket(offs[i + 1] - offs[i]);
+        const int64_t d = offs[i + 1] - offs[i];
+        const uint32_t b = (uint32_t)(d <= lpv_max ? kLpvBucket : degree_bucket(d));
         uint32_t c = 0;
         if (colors) c = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
         keys[i] = c * kNumBuckets + b;
@@ -462,10 +908,17 @@ __global__ void k_color_keys(int64_t nv, const int64_t* __restrict__ colors, int
     }
 }
 
-__global__ void k_key_histogram(const uint32_t* __restrict__ keys, int64_t n, int64_t* __restrict__ hist)
+// Histogram of small keys: per-block counts in LDS, one global atomic per touched bin.
+__global__ void k_key_histogram(const uint32_t* __restrict__ keys, int64_t n, int64_t* __restrict__ hist, int nbins)
 {
+    extern __shared__ unsigned s_bins[];
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_bins[i] = 0;
+    __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd((unsigned long long*)&hist[keys[i]], 1ull);
+        atomicAdd(&s_bins[keys[i]], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+        if (s_bins[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)s_bins[i]);
 }
 
 // Movers in processing order: a vertex that was processed (active, or ET off) and whose
@@ -575,8 +1028,18 @@ __global__ void k_rotate_et2(int64_t nv, int num_iters, double one_minus_delta, 
 struct ColorSweep {
     SweepPlan plan{};
     int64_t fused_blocks = 0;
-    int64_t wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0;
+    int64_t wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0, lpv_off = 0, lpv_n = 0;
 };
+
+// Rows up to this degree take the lane-per-vertex kernel when VITE_AMD_LPV_MAXDEG is set
+// (default 0: every row goes to the group kernels, which measured faster on the RGG
+// workloads because the lane's linear map search is a chain of dependent LDS reads).
+static int lpv_max_degree()
+{
+    const char* e = std::getenv("VITE_AMD_LPV_MAXDEG");
+    int d = e ? std::atoi(e) : 0;
+    return std::max(0, std::min(d, kLpvMaxDeg));
+}
 
 struct Phase {
     vt_ctx* ctx;
@@ -649,9 +1112,9 @@ struct Phase {
         std::vector<int64_t> h(nkeys, 0);
         if (nv) {
             VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv,
-                      ncol > 1 ? d_colors : nullptr, num_color, k0, (uint32_t*)lists.p);
+                      ncol > 1 ? d_colors : nullptr, num_color, lpv_max_degree(), k0, (uint32_t*)lists.p);
             dzero(ctx, hist, 8 * nkeys);
-            VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(nv, 256, 8192), 256, 0, (const uint32_t*)k0, nv, hist);
+            key_histogram(k0, nv, hist, (int)nkeys);
             bool alt = false;
             sort_pairs_u32(ctx, k0, (uint32_t*)lists.p, nv, (int)bits_for(nkeys), k1, v1, &alt);
             if (alt) d2d(ctx, lists.p, v1, 4 * (size_t)nv);
@@ -673,6 +1136,9 @@ struct Phase {
                 } else if (b == kHubBucket - 1) {
                     cs.wide_off = off;
                     cs.wide_n = cnt;
+                } else if (b == kLpvBucket) {
+                    cs.lpv_off = off;
+                    cs.lpv_n = cnt;
                 } else {
                     cs.hub_off = off;
                     cs.hub_n = cnt;
@@ -706,7 +1172,7 @@ struct Phase {
                 VT_LAUNCH(ctx, "color_keys", k_color_keys, grid_for(nv, 256, 8192), 256, 0, nv, d_colors, num_color, k0,
                           (uint32_t*)corder.p);
                 dzero(ctx, ch, 8 * num_color);
-                VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(nv, 256, 8192), 256, 0, (const uint32_t*)k0, nv, ch);
+                key_histogram(k0, nv, ch, (int)num_color);
                 bool alt = false;
                 sort_pairs_u32(ctx, k0, (uint32_t*)corder.p, nv, (int)bits_for(num_color), k1, v1, &alt);
                 if (alt) d2d(ctx, corder.p, v1, 4 * (size_t)nv);
@@ -715,6 +1181,13 @@ struct Phase {
             color_ptr.assign(num_color + 1, 0);
             for (int64_t c = 0; c < num_color; c++) color_ptr[c + 1] = color_ptr[c] + hc[c];
         }
+    }
+
+    void key_histogram(const uint32_t* keys, int64_t n, int64_t* hist, int nbins)
+    {
+        if (nbins > 16384) fail(ctx, VT_ERR_LIMIT, "too many sweep lists (%d)", nbins);
+        VT_LAUNCH(ctx, "key_hist", k_key_histogram, grid_for(n, 256, 2048), 256, sizeof(unsigned) * nbins, keys, n, hist,
+                  nbins);
     }
 
     MoveArgs move_args()
@@ -738,6 +1211,13 @@ struct Phase {
     void sweep(const ColorSweep& cs)
     {
         const MoveArgs a = move_args();
+        if (cs.lpv_n) {
+            const int64_t nblk = std::min<int64_t>((cs.lpv_n + 63) / 64, 16384);
+            const int64_t chunk = ((cs.lpv_n + nblk - 1) / nblk + 63) / 64 * 64;
+            auto kern = k_local_move_lpv<kLpvMaxDeg, kLpvArena, 4>;
+            VT_LAUNCH(ctx, "local_move_lpv", kern, (unsigned)nblk, 64, 0, a, (const int32_t*)(lists.p + cs.lpv_off), cs.lpv_n,
+                      chunk);
+        }
         if (cs.fused_blocks)
             VT_LAUNCH(ctx, "local_move", k_local_move, (unsigned)cs.fused_blocks, kFusedBlock, 0, a,
                       (const int32_t*)lists.p, cs.plan);
