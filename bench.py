@@ -16,6 +16,7 @@ the whole threshold-cycling run, and the reference's own sweep on the host CPU o
 state (oracle/_ref, OpenMP threads), with the GPU targets checked against it.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+    python bench.py --summarize-profile DIR     (condense rocprofv3 CSV output)
 """
 import argparse
 import json
@@ -27,6 +28,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Rows of degree <= 63 run in the fused local-move kernel; degree classes above it have
+# their own launches (mid, wide, hub).
+FUSED_MAX_DEGREE = 63
+# rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) against the bytes a plain 1 GiB
+# device copy really moves: FETCH_SIZE reads half of them, WRITE_SIZE all of them.
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
 METRIC = "local-move edges/s (%HBM roofline); time to final Q ±1e-6; batched graphs/s"
 
 
@@ -62,7 +70,61 @@ def cpu_baseline(dg, st_state, threads, reps, gpu_target):
             "seconds_per_sweep": sec, "target_mismatches_vs_gpu": mism}
 
 
+def summarize_profile(d):
+    """Condenses rocprofv3 CSV output under d (kernel stats, kernel trace, counter
+    collection) into a short text summary: per kernel, dispatches and mean duration, and
+    per (kernel, counter) the mean value per dispatch."""
+    import collections
+    import csv
+    import glob
+
+    def short(name):
+        name = name.split("(")[0] if not name.startswith("void rocprim") else name[:60]
+        return name[:90]
+
+    lines = []
+    for f in sorted(glob.glob(os.path.join(d, "**", "*.csv"), recursive=True)):
+        base = os.path.basename(f)
+        if base.endswith("kernel_stats.csv"):
+            lines.append(f"## {base}: kernel | calls | mean ms | total ms | % of kernel time")
+            with open(f) as fh:
+                for i, row in enumerate(csv.DictReader(fh)):
+                    if i < 25:
+                        lines.append(f"{short(row['Name'])} | {row['Calls']} | {float(row['AverageNs']) / 1e6:.4f} | "
+                                     f"{float(row['TotalDurationNs']) / 1e6:.3f} | {float(row['Percentage']):.2f}")
+        elif base.endswith("counter_collection.csv"):
+            agg = collections.defaultdict(lambda: collections.defaultdict(list))
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    agg[short(row.get("Kernel_Name", "?"))][row.get("Counter_Name", "?")].append(
+                        float(row.get("Counter_Value", 0)))
+            lines.append(f"## {base}: kernel | counter | dispatches | mean per dispatch")
+            for k, cs in agg.items():
+                for c, v in cs.items():
+                    lines.append(f"{k} | {c} | {len(v)} | {sum(v) / len(v):.6g}")
+    return "\n".join(lines) + "\n"
+
+
+def measured_traffic(kernel, workload_tag):
+    """HBM bytes per launch of `kernel` from the committed rocprof counter summary of this
+    workload (FETCH_SIZE x FETCH_SCALE + WRITE_SIZE x WRITE_SCALE), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}_fetch_write.txt")
+    if not os.path.exists(path):
+        return None, None
+    vals = {}
+    for line in open(path):
+        parts = [p.strip() for p in line.split("|")]
+        if len(parts) == 4 and parts[0].endswith("k_" + kernel) and parts[1] in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals[parts[1]] = float(parts[3])
+    if len(vals) != 2:
+        return None, None
+    return 1024.0 * (vals["FETCH_SIZE"] * FETCH_SCALE + vals["WRITE_SIZE"] * WRITE_SCALE), os.path.relpath(path, ROOT)
+
+
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--summarize-profile":
+        sys.stdout.write(summarize_profile(sys.argv[2]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -124,11 +186,18 @@ def main():
     state = st.state()
     st.end(want_cvect=False)
 
-    alg_bytes = 24 * E + 56 * V
+    # the dominant kernel's own share: rows of degree <= FUSED_MAX_DEGREE
+    deg = np.diff(dg.offsets())
+    fused = deg <= FUSED_MAX_DEGREE
+    Vf, Ef = int(np.count_nonzero(fused)), int(deg[fused].sum())
+    alg_bytes = 24 * Ef + 56 * Vf
     achieved = alg_bytes / (kavg * 1e-3) / 1e9 if kavg > 0 else 0.0
     value = E / (ms_per_step * 1e-3)
-    log(f"sweep: {ms_per_step:.3f} ms/step wall, kernel {kavg:.3f} ms avg over {kcount} launches, "
-        f"{achieved:.1f} GB/s algorithmic ({100 * achieved / HBM_PEAK_GBS:.1f}% of peak), {value / 1e9:.2f} G edges/s")
+    log(f"sweep: {ms_per_step:.3f} ms/step wall, kernel {kavg:.3f} ms avg over {kcount} launches "
+        f"({Vf} of {V} rows, {Ef} of {E} entries), {achieved:.1f} GB/s algorithmic "
+        f"({100 * achieved / HBM_PEAK_GBS:.1f}% of peak), {value / 1e9:.2f} G edges/s")
+    tag = f"rgg_n{V}"
+    traffic, traffic_src = measured_traffic("local_move", tag)
 
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -139,9 +208,12 @@ def main():
                    "n": V, "directed_entries": E, "seed": args.seed, "ids": "int64", "weights": "f64",
                    "threshold": 1e-3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "local_move", "kernel_avg_ms": kavg, "algorithmic_bytes_per_launch": alg_bytes,
-                     "bytes_formula": "24E + 56V"},
+                     "bytes_formula": "24E + 56V over the kernel's rows (degree <= 63)",
+                     "kernel_rows": Vf, "kernel_entries": Ef,
+                     "traffic_source": traffic_src,
+                     "traffic_formula": f"FETCH_SIZE x {FETCH_SCALE} + WRITE_SIZE x {WRITE_SCALE} (KiB per dispatch)"},
     }
 
     if not args.no_ttq:

@@ -63,6 +63,25 @@ def _unit_grid(side=40):
     return po.from_edge_list(n, pairs)
 
 
+def _dense_geometric(seed=21, n=2400, mean_deg=70.0, extra=3000):
+    """Geometric graph with degrees spread over roughly 35-110, so most rows fall in the
+    32-63 and 64-127 degree classes (the two largest group sizes of the local move), plus
+    random long edges."""
+    rng = np.random.default_rng(seed)
+    x, y = rng.random(n), rng.random(n)
+    r = np.sqrt(mean_deg / (np.pi * n))
+    d = np.sqrt((x[:, None] - x[None, :]) ** 2 + (y[:, None] - y[None, :]) ** 2)
+    i, j = np.nonzero(np.triu(d <= r, 1))
+    w = d[i, j]
+    a = rng.integers(0, n, size=extra)
+    b = rng.integers(0, n, size=extra)
+    keep = a != b
+    pairs = np.concatenate([np.stack([i, j], 1), np.stack([np.minimum(a, b), np.maximum(a, b)], 1)[keep]])
+    wts = np.concatenate([w, rng.uniform(0.0, 1.0, size=int(keep.sum()))])
+    pairs, first = np.unique(pairs, axis=0, return_index=True)
+    return po.from_edge_list(n, pairs, wts[first])
+
+
 @functools.lru_cache(maxsize=None)
 def graph(name: str) -> po.Graph:
     if name == "rgg":
@@ -79,10 +98,12 @@ def graph(name: str) -> po.Graph:
         return _multi_graph()
     if name == "grid":
         return _unit_grid()
+    if name == "dense":
+        return _dense_geometric()
     raise KeyError(name)
 
 
-ALL = ["rgg", "rgg_plain", "ba", "vite_rgg", "hub", "multi", "grid"]
+ALL = ["rgg", "rgg_plain", "ba", "vite_rgg", "hub", "multi", "grid", "dense"]
 
 
 def random_state(g: po.Graph, seed: int = 0):

@@ -242,6 +242,12 @@ class DeviceGraph:
         except Exception:
             pass
 
+    def offsets(self) -> np.ndarray:
+        """The CSR row offsets only (nv + 1 entries)."""
+        offs = np.zeros(self.nv + 1, np.int64)
+        self.ctx.check(_lib.vt_graph_download(self.ctx.h, self.h, _p(offs), None))
+        return offs
+
     def download(self) -> CSR:
         offs = np.zeros(self.nv + 1, np.int64)
         edges = np.zeros(self.ne, EDGE_DTYPE)

@@ -290,63 +290,85 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 
 // Insert-or-find without an occupancy list: CAS first (one LDS round trip whether the key
 // is new or already present).
-__device__ __forceinline__ int32_t lds_slot(int32_t* keys, uint32_t capmask, int32_t c)
-{
-    uint32_t h = (slot_hash(c) >> 7) & capmask;
-    for (;;) {
-        const int32_t old = atomicCAS(&keys[h], -1, c);
-        if (old == -1 || old == c) return (int32_t)h;
-        h = (h + 1) & capmask;
-    }
-}
+// Table of one group (G lanes, one vertex at a time) in LDS.
+template <int G>
+struct GroupTable {
+    using mask_t = typename MaskT<G>::type;
+    int32_t* keys;    // CAP slots, -1 = empty
+    double* vals;     // CAP, sentinel = no value yet
+    mask_t* masks;    // CAP: lanes of the current chunk holding the slot's community
+    uint8_t* occ;     // 2G: occupied slots in insertion order
+    double* wst;      // G staged chunk weights; wst[-1] holds -0.0
+    uint32_t capmask;
+};
 
-// Folds the weights of the lanes in `rest` (ascending lane = edge order) onto acc, reading
-// the staged weights eight at a time so the LDS latency is paid once per eight terms.
-template <class mask_t>
+// Folds the staged weights of the lanes in `rest` (ascending lane = edge order) onto acc,
+// U terms per round so the LDS latency is paid once per round.  Rounds past the last term
+// read wst[-1] = -0.0, and x + (-0.0) == x for every x, so no select is needed.
+template <class mask_t, int U>
 __device__ __forceinline__ double fold_rest(double acc, mask_t rest, const double* wst)
 {
     while (rest) {
-        int k[8];
-        bool ok[8];
+        int k[U];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            ok[u] = rest != 0;
-            k[u] = ok[u] ? (int)(sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)rest)
-                                                     : __builtin_ctz((unsigned)rest))
-                         : 0;
-            rest = ok[u] ? (mask_t)(rest & (rest - 1)) : rest;
+        for (int u = 0; u < U; u++) {
+            k[u] = rest ? (int)(sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)rest)
+                                                    : __builtin_ctz((unsigned)rest))
+                        : -1;
+            rest = (mask_t)(rest & (rest - 1));
         }
-        double w[8];
+        double w[U];
 #pragma unroll
-        for (int u = 0; u < 8; u++) w[u] = wst[k[u]];
+        for (int u = 0; u < U; u++) w[u] = wst[k[u]];
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-            if (ok[u]) acc = acc + w[u];
+        for (int u = 0; u < U; u++) acc = acc + w[u];
     }
     return acc;
 }
 
+#ifndef VT_FOLD_UNROLL
+#define VT_FOLD_UNROLL 4
+#endif
+
 // One chunk of at most G edges of a row already in registers: insert / find each
-// community, elect the lowest lane per community, and let it fold the chunk's weights for
-// that community in edge order onto the running counter.  A counter created here starts as
-// the first weight, or 0.0 + w for the vertex's own community, whose counter the reference
-// seeds with 0.0 (louvain.cpp:2326-2327, 2456-2462).
+// community (recording new slots in occ), elect the lowest lane per community, and let it
+// fold the chunk's weights for that community in edge order onto the running counter.  A
+// counter created here starts as the first weight, or 0.0 + w for the vertex's own
+// community, whose counter the reference seeds with 0.0 (louvain.cpp:2326-2327, 2456-2462).
+// `nocc` (uniform over the group) counts the slots recorded so far.
 template <int G>
-__device__ __forceinline__ void fold_chunk(const Table<G, false>& t, double* ecc_cell, int32_t c, int32_t cc, double w,
-                                           bool valid, bool self, int lane, int gbase, double& selfLoop)
+__device__ __forceinline__ void fold_chunk(const GroupTable<G>& t, double* ecc_cell, int32_t c, int32_t cc, double w,
+                                           bool valid, bool self, int lane, int gbase, double& selfLoop, int& nocc)
 {
     using mask_t = typename MaskT<G>::type;
     const mask_t gmask = (G == 64) ? (mask_t)~0ull : (mask_t)((1ull << G) - 1);
-    const int32_t slot = valid ? lds_slot(t.keys, t.capmask, c) : -1;
+    const mask_t below = (mask_t)(((mask_t)1 << lane) - 1);
+    int32_t slot = -1;
+    bool fresh = false;
+    if (valid) {
+        uint32_t h = (slot_hash(c) >> 7) & t.capmask;
+        for (;;) {
+            const int32_t old = atomicCAS(&t.keys[h], -1, c);
+            if (old == -1 || old == c) {
+                fresh = old == -1;
+                break;
+            }
+            h = (h + 1) & t.capmask;
+        }
+        slot = (int32_t)h;
+    }
+    const mask_t fm = (mask_t)((__ballot(fresh) >> gbase) & gmask);
+    if (fresh) t.occ[nocc + __popcll((unsigned long long)(fm & below))] = (uint8_t)slot;
+    nocc += __popcll((unsigned long long)fm);
     t.wst[lane] = w;
     if (valid) atomicOr(&t.masks[slot], (mask_t)1 << lane);
     wave_sync();
     const mask_t m = valid ? t.masks[slot] : (mask_t)0;
     wave_sync();
-    if (valid && (m & (((mask_t)1 << lane) - 1)) == 0) {
+    if (valid && (m & below) == 0) {
         double acc = t.vals[slot];
         acc = is_sent(acc) ? (c == cc ? 0.0 + w : w) : acc + w;
-        acc = fold_rest<mask_t>(acc, (mask_t)(m & (m - 1)), t.wst);
+        acc = fold_rest<mask_t, VT_FOLD_UNROLL>(acc, (mask_t)(m & (m - 1)), t.wst);
         t.vals[slot] = acc;
         t.masks[slot] = 0;
         if (c == cc) *ecc_cell = acc;
@@ -441,11 +463,11 @@ __device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, 
 }
 
 template <int G, int CAP, int BLOCK, bool PIPE>
-__device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Table<G, false>& t, double* ecc_cell,
+__device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const GroupTable<G>& t, double* ecc_cell,
                                                   const int32_t* __restrict__ list, int64_t n, int64_t idx,
                                                   int64_t stride, int lane, int gbase, unsigned* inactive_local)
 {
-    static_assert(CAP == 4 * G, "four slots per lane");
+    static_assert(CAP == 4 * G && CAP <= 256, "occ holds 8-bit slot indexes");
     if (idx >= n) return;
     VHeader h = load_header(a, list[idx], true);
     int32_t vn = (idx + stride < n) ? list[idx + stride] : 0;
@@ -486,54 +508,41 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Table
             const int64_t ccSize = a.csize[cc];
             if (lane == 0) *ecc_cell = 0.0;
             double selfLoop = 0.0;
-            fold_chunk<G>(t, ecc_cell, r.c0, (int32_t)cc, r.w0, r.ok0, r.self0, lane,
-                          gbase, selfLoop);
+            int nocc = 0;
+            fold_chunk<G>(t, ecc_cell, r.c0, (int32_t)cc, r.w0, r.ok0, r.self0, lane, gbase, selfLoop, nocc);
             if (h.deg > G)
-                fold_chunk<G>(t, ecc_cell, r.c1, (int32_t)cc, r.w1, r.ok1, r.self1,
-                              lane, gbase, selfLoop);
-            // scan this lane's four slots, clearing them as they are read
-            int32_t ck[4];
-            double ey[4], ay[4];
-            bool ok[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int sl = lane + u * G;
-                ck[u] = t.keys[sl];
-                ey[u] = t.vals[sl];
-            }
+                fold_chunk<G>(t, ecc_cell, r.c1, (int32_t)cc, r.w1, r.ok1, r.self1, lane, gbase, selfLoop, nocc);
             const double ecc = *ecc_cell;
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int sl = lane + u * G;
-                t.keys[sl] = -1;
-                t.vals[sl] = sent_value();
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                ok[u] = ck[u] != -1 && (int64_t)ck[u] != cc;
-                ay[u] = ok[u] ? a.cdeg[ck[u]] : 0.0;
-            }
             // the next vertex's neighbour communities, before waiting on this one's gathers
             if (has_next) row_issue_gathers(a, hn.v, rn);
-            // distGetMaxIndex (louvain.cpp:2246-2276)
+            // distGetMaxIndex (louvain.cpp:2246-2276) over the occupied slots, clearing them
             const double kv = h.kv;
             const double eix = ecc - selfLoop;
             const double ax = ccDeg - kv;
             double bg = 0.0;
-            long long bc = LLONG_MAX;
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (ok[u]) {
-                    const double gain = 2.0 * (ey[u] - eix) - 2.0 * kv * (ay[u] - ax) * constant;
-                    if (gain > 0.0 && (gain > bg || (gain == bg && (long long)ck[u] < bc))) {
+            int32_t bc = INT_MAX;
+            for (int base = 0; base < nocc; base += G) {
+                const bool ok = base + lane < nocc;
+                const int sl = ok ? t.occ[base + lane] : 0;
+                const int32_t ck = ok ? t.keys[sl] : -1;
+                const double ey = t.vals[sl];
+                if (ok) {
+                    t.keys[sl] = -1;
+                    t.vals[sl] = sent_value();
+                }
+                const bool cand = ok && (int64_t)ck != cc;
+                const double ay = cand ? a.cdeg[ck] : 0.0;
+                if (cand) {
+                    const double gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
+                    if (gain > 0.0 && (gain > bg || (gain == bg && ck < bc))) {
                         bg = gain;
-                        bc = ck[u];
+                        bc = ck;
                     }
                 }
             }
             for (int o = G / 2; o > 0; o >>= 1) {
                 const double og = __shfl_xor(bg, o, G);
-                const long long oc = __shfl_xor(bc, o, G);
+                const int32_t oc = __shfl_xor(bc, o, G);
                 if (og > bg || (og == bg && oc < bc)) {
                     bg = og;
                     bc = oc;
@@ -562,7 +571,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Table
 // LDS carve-out for BLOCK/G groups of CAP-slot tables.
 constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-template <int G, int CAP, int BLOCK, bool OCC = true>
+template <int G, int CAP, int BLOCK>
 struct LdsLayout {
     using mask_t = typename MaskT<G>::type;
     static constexpr int NG = BLOCK / G;
@@ -570,11 +579,25 @@ struct LdsLayout {
     static constexpr size_t vals = align16(keys + sizeof(int32_t) * NG * CAP);
     static constexpr size_t masks = align16(vals + sizeof(double) * NG * CAP);
     static constexpr size_t occ = align16(masks + sizeof(mask_t) * NG * CAP);
-    static constexpr size_t wst = align16(occ + (OCC ? sizeof(int32_t) * NG * CAP : 0));
+    static constexpr size_t wst = align16(occ + sizeof(int32_t) * NG * CAP);
     static constexpr size_t nocc = align16(wst + sizeof(double) * NG * G);
     static constexpr size_t cslot = align16(nocc + sizeof(int32_t) * NG);
     static constexpr size_t ecc = align16(cslot + sizeof(int32_t) * NG);
     static constexpr size_t bytes = align16(ecc + sizeof(double) * NG);
+};
+
+// LDS carve-out of the two-chunk group buckets.
+template <int G, int CAP, int BLOCK>
+struct GroupLds {
+    using mask_t = typename MaskT<G>::type;
+    static constexpr int NG = BLOCK / G;
+    static constexpr size_t keys = 0;
+    static constexpr size_t vals = align16(keys + sizeof(int32_t) * NG * CAP);
+    static constexpr size_t masks = align16(vals + sizeof(double) * NG * CAP);
+    static constexpr size_t wst = align16(masks + sizeof(mask_t) * NG * CAP);
+    static constexpr size_t ecc = align16(wst + sizeof(double) * NG * (G + 1));
+    static constexpr size_t occ = align16(ecc + sizeof(double) * NG);
+    static constexpr size_t bytes = align16(occ + (size_t)NG * 2 * G);
 };
 
 // Runs one degree bucket inside a block: block `blk` of `nblk` owns list entries
@@ -583,45 +606,63 @@ template <int G, int CAP, int BLOCK, bool TWO>
 __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const int32_t* __restrict__ list, int64_t n,
                                            int64_t blk, int64_t nblk, unsigned* inactive)
 {
-    using L = LdsLayout<G, CAP, BLOCK, !TWO>;
-    using mask_t = typename L::mask_t;
-    constexpr int NG = L::NG;
-    int32_t* keys = (int32_t*)(lds + L::keys);
-    double* vals = (double*)(lds + L::vals);
-    mask_t* masks = (mask_t*)(lds + L::masks);
-    int32_t* occ = (int32_t*)(lds + L::occ);
-    double* wst = (double*)(lds + L::wst);
-    int32_t* nocc = (int32_t*)(lds + L::nocc);
-    int32_t* cslot = (int32_t*)(lds + L::cslot);
-    for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
-        keys[i] = -1;
-        vals[i] = sent_value();
-        masks[i] = 0;
-    }
-    if (threadIdx.x < NG) nocc[threadIdx.x] = 0;
-    __syncthreads();
     const int g = threadIdx.x / G;
     const int lane = threadIdx.x % G;
     const int gbase = (threadIdx.x % 64) / G * G;
-    Table<G, false> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, occ + g * CAP, wst + g * G, nullptr,
-                      nocc + g, cslot + g, (uint32_t)(CAP - 1)};
-    const int64_t stride = nblk * NG;
-    int64_t idx = blk * NG + g;
+    const int64_t stride = nblk * (BLOCK / G);
+    int64_t idx = blk * (BLOCK / G) + g;
     if constexpr (TWO) {
+        using L = GroupLds<G, CAP, BLOCK>;
+        using mask_t = typename L::mask_t;
+        constexpr int NG = L::NG;
+        int32_t* keys = (int32_t*)(lds + L::keys);
+        double* vals = (double*)(lds + L::vals);
+        mask_t* masks = (mask_t*)(lds + L::masks);
+        double* wst = (double*)(lds + L::wst);
+        for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
+            keys[i] = -1;
+            vals[i] = sent_value();
+            masks[i] = 0;
+        }
+        if (threadIdx.x < NG) wst[threadIdx.x * (G + 1)] = -0.0;
+        __syncthreads();
+        GroupTable<G> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, (uint8_t*)(lds + L::occ) + g * 2 * G,
+                        wst + g * (G + 1) + 1, (uint32_t)(CAP - 1)};
         double* ecc_cell = (double*)(lds + L::ecc) + g;
         run_two_pipelined<G, CAP, BLOCK, kPipeRows>(a, t, ecc_cell, list, n, idx, stride, lane, gbase, inactive);
     } else {
+        using L = LdsLayout<G, CAP, BLOCK>;
+        using mask_t = typename L::mask_t;
+        constexpr int NG = L::NG;
+        int32_t* keys = (int32_t*)(lds + L::keys);
+        double* vals = (double*)(lds + L::vals);
+        mask_t* masks = (mask_t*)(lds + L::masks);
+        int32_t* occ = (int32_t*)(lds + L::occ);
+        double* wst = (double*)(lds + L::wst);
+        int32_t* nocc = (int32_t*)(lds + L::nocc);
+        int32_t* cslot = (int32_t*)(lds + L::cslot);
+        for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
+            keys[i] = -1;
+            vals[i] = sent_value();
+            masks[i] = 0;
+        }
+        if (threadIdx.x < NG) nocc[threadIdx.x] = 0;
+        __syncthreads();
+        Table<G, false> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, occ + g * CAP, wst + g * G, nullptr,
+                          nocc + g, cslot + g, (uint32_t)(CAP - 1)};
         for (; idx < n; idx += stride) move_vertex<G, false>(a, t, list[idx], lane, gbase, inactive);
     }
 }
 
-// Buckets 0-4 (degree <= 127) of one sweep in a single launch: blocks [bbeg[b], bbeg[b+1])
-// run bucket b over its list range.
-constexpr int kFusedBuckets = 5;
+// Buckets 0-3 (degree <= 63) of one sweep in a single launch: blocks [bbeg[b], bbeg[b+1])
+// run bucket b over its list range.  Bucket 4 (64 <= degree <= 127) has its own launch so
+// that its larger tables do not set the LDS footprint, and so the occupancy, of this one.
+constexpr int kFusedBuckets = 4;
 constexpr int kFusedBlock = 256;
-constexpr size_t kFusedLds = std::max({LdsLayout<4, 16, 256, false>::bytes, LdsLayout<8, 32, 256, false>::bytes,
-                                       LdsLayout<16, 64, 256, false>::bytes, LdsLayout<32, 128, 256, false>::bytes,
-                                       LdsLayout<64, 256, 256, false>::bytes});
+constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256>::bytes, GroupLds<8, 32, 256>::bytes,
+                                       GroupLds<16, 64, 256>::bytes, GroupLds<32, 128, 256>::bytes});
+constexpr size_t kMidLds = GroupLds<64, 256, 256>::bytes;
+static_assert(kFusedLds <= 20480, "eight 256-lane blocks per CU");
 
 struct SweepPlan {
     int64_t bbeg[kFusedBuckets + 1];  // block ranges
@@ -646,9 +687,19 @@ k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
         case 0: run_bucket<4, 16, 256, true>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
         case 1: run_bucket<8, 32, 256, true>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
         case 2: run_bucket<16, 64, 256, true>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
-        case 3: run_bucket<32, 128, 256, true>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
-        default: run_bucket<64, 256, 256, true>(a, lds, list, plan.n[4], blk, nblk, &s_inactive); break;
+        default: run_bucket<32, 128, 256, true>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
+}
+
+// Bucket 4 (64 <= degree <= 127): four 64-lane groups per block, 256-slot tables.
+__global__ void __launch_bounds__(256) k_local_move_mid(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
+{
+    __shared__ __align__(16) char lds[kMidLds];
+    __shared__ unsigned s_inactive;
+    if (threadIdx.x == 0) s_inactive = 0;
+    run_bucket<64, 256, 256, true>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
@@ -1024,11 +1075,11 @@ __global__ void k_rotate_et2(int64_t nv, int num_iters, double one_minus_delta, 
 // Host side
 // ------------------------------------------------------------------------------------------
 
-// One color's sweep: buckets 0-4 fused into one launch, then the wide and hub buckets.
+// One color's sweep: buckets 0-3 fused into one launch, then the mid, wide and hub buckets.
 struct ColorSweep {
     SweepPlan plan{};
     int64_t fused_blocks = 0;
-    int64_t wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0, lpv_off = 0, lpv_n = 0;
+    int64_t mid_off = 0, mid_n = 0, wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0, lpv_off = 0, lpv_n = 0;
 };
 
 // Rows up to this degree take the lane-per-vertex kernel when VITE_AMD_LPV_MAXDEG is set
@@ -1122,7 +1173,7 @@ struct Phase {
         }
         sweeps.assign(ncol, ColorSweep{});
         int64_t off = 0, max_hub = 0;
-        constexpr int kGroups[kFusedBuckets] = {256 / 4, 256 / 8, 256 / 16, 256 / 32, 256 / 64};
+        constexpr int kGroups[kFusedBuckets] = {256 / 4, 256 / 8, 256 / 16, 256 / 32};
         for (int64_t c = 0; c < ncol; c++) {
             ColorSweep& cs = sweeps[c];
             int64_t blocks = 0;
@@ -1133,6 +1184,9 @@ struct Phase {
                     cs.plan.off[b] = off;
                     cs.plan.n[b] = cnt;
                     blocks += std::min<int64_t>((cnt + kGroups[b] - 1) / kGroups[b], 16384);
+                } else if (b == kFusedBuckets) {
+                    cs.mid_off = off;
+                    cs.mid_n = cnt;
                 } else if (b == kHubBucket - 1) {
                     cs.wide_off = off;
                     cs.wide_n = cnt;
@@ -1221,6 +1275,9 @@ struct Phase {
         if (cs.fused_blocks)
             VT_LAUNCH(ctx, "local_move", k_local_move, (unsigned)cs.fused_blocks, kFusedBlock, 0, a,
                       (const int32_t*)lists.p, cs.plan);
+        if (cs.mid_n)
+            VT_LAUNCH(ctx, "local_move_mid", k_local_move_mid, (unsigned)std::min<int64_t>((cs.mid_n + 3) / 4, 16384), 256,
+                      0, a, (const int32_t*)(lists.p + cs.mid_off), cs.mid_n);
         if (cs.wide_n)
             VT_LAUNCH(ctx, "local_move_wide", k_local_move_wide, (unsigned)std::min<int64_t>(cs.wide_n, 8192), 64, 0, a,
                       (const int32_t*)(lists.p + cs.wide_off), cs.wide_n);
