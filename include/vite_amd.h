@@ -21,6 +21,7 @@
  *   vt_gen_rgg, vt_gen_ba  the workload generators (no reference counterpart: Vite's RGG uses
  *                          another radius and an O(n^2) pair test; its BA needs NetworKit)
  *   vt_run                 the phase loop of main.cpp:242-513
+ *   vt_batch_*             the same loop over many graphs at once (no reference counterpart)
  */
 #ifndef VITE_AMD_H
 #define VITE_AMD_H
@@ -80,6 +81,8 @@ int vt_graph_free(vt_ctx* ctx, vt_graph* g);
 #define VT_FLAG_SEQUENTIAL_SUMS 1  /* 2m and modularity as left folds over vertex order,
                                       as one OpenMP thread of the reference computes them
                                       (slow; for exact whole-run replay on small graphs) */
+#define VT_FLAG_NO_GRAPH 2         /* vt_batch_run: launch each iteration's kernels directly
+                                      instead of replaying captured HIP graphs            */
 
 typedef struct {
     double lower;          /* `lower` argument                                          */
@@ -192,6 +195,27 @@ typedef struct {
 
 /* Runs the phase loop on `g` (not modified). */
 int vt_run(vt_ctx* ctx, const vt_graph* g, const vt_run_params* p, vt_run_result* res);
+
+/* ---- batched form (no reference counterpart: Vite runs one graph per job) -------------- */
+/* Many independent graphs run the phase loop together as one device-resident disjoint
+ * union: each Louvain iteration of every graph runs in the same launches (replayed from
+ * captured HIP graphs), each graph leaving its phase on its own test, and all members are
+ * coarsened together between phases.  Per graph, every call's return value and iteration
+ * count and the final modularity equal vt_run on that graph alone.  Plain Louvain only
+ * (threshold, threshold cycling, one phase); coloring and early termination are refused. */
+typedef struct vt_batch vt_batch;
+typedef struct {
+    int32_t phases;   /* batched phases run (rounds of the phase loop)             */
+    int32_t rounds;   /* batched iterations over all phases                        */
+    double seconds;   /* wall time of the whole batched run                        */
+} vt_batch_stats;
+/* Copies n device graphs into a batch; the graphs stay the caller's.  Each needs at least one
+ * edge (with 2m = 0 modularity is undefined and the reference's loop never ends). */
+int vt_batch_create(vt_ctx* ctx, int32_t n, vt_graph* const* graphs, vt_batch** out);
+int vt_batch_info(const vt_batch* b, int32_t* n, int64_t* total_nv, int64_t* total_ne);
+/* results: n entries, in the order the graphs were given.  stats may be NULL. */
+int vt_batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* p, vt_run_result* results, vt_batch_stats* stats);
+int vt_batch_free(vt_ctx* ctx, vt_batch* b);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ VT_ERR_STATE = -5
 
 VT_MODE_PLAIN, VT_MODE_COLOR, VT_MODE_VORDER = 0, 1, 2
 VT_FLAG_SEQUENTIAL_SUMS = 1
+VT_FLAG_NO_GRAPH = 2
 MAX_LEVELS = 256
 # First word std::seed_seq{1} generates: the LCG start of Vite's generator (utils.hpp:163-170).
 VITE_LCG_X0 = 1967017404
@@ -66,6 +67,10 @@ class _RunResult(C.Structure):
 
 
 # Every symbol include/vite_amd.h declares, with its ctypes signature.
+class _BatchStats(C.Structure):
+    _fields_ = [("phases", C.c_int32), ("rounds", C.c_int32), ("seconds", C.c_double)]
+
+
 _SIGS = {
     "vt_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "vt_ctx_destroy": (C.c_int, [C.c_void_p]),
@@ -96,6 +101,11 @@ _SIGS = {
     "vt_gen_ba": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
     "vt_gen_vite_rgg": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p)]),
     "vt_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_RunParams), C.POINTER(_RunResult)]),
+    "vt_batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "vt_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "vt_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_RunParams), C.POINTER(_RunResult),
+                               C.POINTER(_BatchStats)]),
+    "vt_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
 
 
@@ -326,15 +336,60 @@ class DeviceGraph:
 
     def run(self, threshold=1e-6, threshold_cycling=False, et_type=0, et_delta=1.0, coloring=False,
             vertex_order=False, max_colors=8, single_color_iter=False, one_phase=False, sequential=False):
-        rp = _RunParams(threshold, int(threshold_cycling), et_type, et_delta, int(coloring), int(vertex_order),
-                        max_colors, int(single_color_iter), int(one_phase),
-                        VT_FLAG_SEQUENTIAL_SUMS if sequential else 0)
+        rp = _run_params(threshold, threshold_cycling, et_type, et_delta, coloring, vertex_order, max_colors,
+                         single_color_iter, one_phase, sequential)
         res = _RunResult()
         self.ctx.check(_lib.vt_run(self.ctx.h, self.h, C.byref(rp), C.byref(res)))
-        n, m = res.nlevels, res.ncalls
-        return RunResult(list(res.level_q[:n]), list(res.level_iters[:n]), list(res.level_nv[:n]),
-                         list(res.level_ne[:n]), list(res.call_q[:m]), list(res.call_iters[:m]), res.final_q,
-                         res.total_iters, res.num_colors, res.clustering_seconds, res.total_seconds)
+        return _run_result(res)
+
+
+def _run_params(threshold=1e-6, threshold_cycling=False, et_type=0, et_delta=1.0, coloring=False,
+                vertex_order=False, max_colors=8, single_color_iter=False, one_phase=False, sequential=False,
+                flags=0):
+    return _RunParams(threshold, int(threshold_cycling), et_type, et_delta, int(coloring), int(vertex_order),
+                      max_colors, int(single_color_iter), int(one_phase),
+                      (VT_FLAG_SEQUENTIAL_SUMS if sequential else 0) | flags)
+
+
+def _run_result(res) -> "RunResult":
+    n, m = res.nlevels, res.ncalls
+    return RunResult(list(res.level_q[:n]), list(res.level_iters[:n]), list(res.level_nv[:n]),
+                     list(res.level_ne[:n]), list(res.call_q[:m]), list(res.call_iters[:m]), res.final_q,
+                     res.total_iters, res.num_colors, res.clustering_seconds, res.total_seconds)
+
+
+class Batch:
+    """Many device graphs run through the phase loop together (vt_batch_*)."""
+
+    def __init__(self, ctx: "Context", graphs):
+        self.ctx = ctx
+        self.n = len(graphs)
+        arr = (C.c_void_p * self.n)(*[g.h for g in graphs])
+        h = C.c_void_p()
+        ctx.check(_lib.vt_batch_create(ctx.h, self.n, arr, C.byref(h)))
+        self.h = h
+        n, nv, ne = C.c_int32(), C.c_int64(), C.c_int64()
+        ctx.check(_lib.vt_batch_info(self.h, C.byref(n), C.byref(nv), C.byref(ne)))
+        self.total_nv, self.total_ne = nv.value, ne.value
+
+    def run(self, use_graph=True, **kw):
+        """Returns (list of RunResult in graph order, stats dict)."""
+        rp = _run_params(flags=0 if use_graph else VT_FLAG_NO_GRAPH, **kw)
+        res = (_RunResult * self.n)()
+        st = _BatchStats()
+        self.ctx.check(_lib.vt_batch_run(self.ctx.h, self.h, C.byref(rp), res, C.byref(st)))
+        return [_run_result(r) for r in res], dict(phases=st.phases, rounds=st.rounds, seconds=st.seconds)
+
+    def free(self):
+        if self.h:
+            h, self.h = self.h, None
+            self.ctx.check(_lib.vt_batch_free(self.ctx.h, h))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class PhaseState:

@@ -240,6 +240,130 @@ void seq_sum2(vt_ctx* ctx, const double* a, const double* b, int64_t n, double* 
     VT_LAUNCH(ctx, "seq_sum2", k_seq_sum2, 1, 64, 0, a, b, n, d_out2);
 }
 
+// ---- per-graph sums over a batch of disjoint graphs ----------------------------------------
+// A graph of n vertices at offset voff[g] is reduced with exactly the blocking and trees
+// tree_sum2 uses on an n-vertex array (same grid, chunk, strided thread folds and block
+// tree; same final block), so its sums in a batch are bit-identical to its sums alone.
+
+__global__ void __launch_bounds__(kSumBlock) k_seg_sum2_partial(const double* __restrict__ a,
+                                                                const double* __restrict__ b,
+                                                                const int64_t* __restrict__ voff,
+                                                                const int32_t* __restrict__ bgraph,
+                                                                const int32_t* __restrict__ bsub,
+                                                                const int64_t* __restrict__ chunk,
+                                                                double2* __restrict__ part)
+{
+    __shared__ double sa[kSumBlock], sb[kSumBlock];
+    const int32_t g = bgraph[blockIdx.x];
+    const int64_t base = voff[g], n = voff[g + 1] - base, ch = chunk[g];
+    const int64_t lo = (int64_t)bsub[blockIdx.x] * ch;
+    const int64_t hi = (lo + ch < n) ? lo + ch : n;
+    double xa = 0.0, xb = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kSumBlock) {
+        xa = xa + a[base + i];
+        if (b) { const double v = b[base + i]; xb = xb + v * v; }
+    }
+    sa[threadIdx.x] = xa;
+    sb[threadIdx.x] = xb;
+    __syncthreads();
+    for (int s = kSumBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + s];
+            sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sa[0], sb[0]);
+}
+
+__global__ void __launch_bounds__(kSumBlock) k_seg_sum2_final(const double2* __restrict__ part,
+                                                              const int64_t* __restrict__ pbase,
+                                                              double* __restrict__ out)
+{
+    __shared__ double sa[kSumBlock], sb[kSumBlock];
+    const int32_t g = blockIdx.x;
+    const double2* p = part + pbase[g];
+    const int np = (int)(pbase[g + 1] - pbase[g]);
+    double xa = 0.0, xb = 0.0;
+    for (int i = threadIdx.x; i < np; i += kSumBlock) { xa = xa + p[i].x; xb = xb + p[i].y; }
+    sa[threadIdx.x] = xa;
+    sb[threadIdx.x] = xb;
+    __syncthreads();
+    for (int s = kSumBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + s];
+            sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[2 * g] = sa[0]; out[2 * g + 1] = sb[0]; }
+}
+
+// Left folds per graph, as k_seq_sum2 forms them for one graph.
+__global__ void k_seg_seq_sum2(const double* __restrict__ a, const double* __restrict__ b,
+                               const int64_t* __restrict__ voff, int32_t ng, double* __restrict__ out)
+{
+    for (int32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+        double xa = 0.0, xb = 0.0;
+        for (int64_t i = voff[g]; i < voff[g + 1]; i++) {
+            xa = xa + a[i];
+            if (b) { const double v = b[i]; xb = xb + v * v; }
+        }
+        out[2 * g] = 0.0 + xa;
+        out[2 * g + 1] = 0.0 + xb;
+    }
+}
+
+void seg_sums_plan(vt_ctx* ctx, const std::vector<int64_t>& voff, SegSums& s)
+{
+    const int32_t ng = (int32_t)voff.size() - 1;
+    std::vector<int32_t> bgraph, bsub;
+    std::vector<int64_t> chunk(ng), pbase(ng + 1, 0);
+    for (int32_t g = 0; g < ng; g++) {
+        const int64_t n = voff[g + 1] - voff[g];
+        int64_t grid = (n + 4 * kSumBlock - 1) / (4 * kSumBlock);
+        if (grid < 1) grid = 1;
+        if (grid > kSumGrid) grid = kSumGrid;
+        const int64_t ch = (n + grid - 1) / grid;
+        chunk[g] = ch > 0 ? ch : 1;
+        for (int64_t b = 0; b < grid; b++) {
+            bgraph.push_back(g);
+            bsub.push_back((int32_t)b);
+        }
+        pbase[g + 1] = pbase[g] + grid;
+    }
+    s.ng = ng;
+    s.nblocks = (int64_t)bgraph.size();
+    s.voff = DBuf<int64_t>(ctx, voff.size());
+    s.bgraph = DBuf<int32_t>(ctx, bgraph.size());
+    s.bsub = DBuf<int32_t>(ctx, bsub.size());
+    s.chunk = DBuf<int64_t>(ctx, chunk.size());
+    s.pbase = DBuf<int64_t>(ctx, pbase.size());
+    s.part = DBuf<double2>(ctx, bgraph.size());
+    h2d(ctx, s.voff.p, voff.data(), 8 * voff.size());
+    h2d(ctx, s.bgraph.p, bgraph.data(), 4 * bgraph.size());
+    h2d(ctx, s.bsub.p, bsub.data(), 4 * bsub.size());
+    h2d(ctx, s.chunk.p, chunk.data(), 8 * chunk.size());
+    h2d(ctx, s.pbase.p, pbase.data(), 8 * pbase.size());
+}
+
+void seg_tree_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b, double* d_out)
+{
+    if (s.ng == 0) return;
+    VT_LAUNCH(ctx, "seg_sum2_partial", k_seg_sum2_partial, (unsigned)s.nblocks, kSumBlock, 0, a, b,
+              (const int64_t*)s.voff.p, (const int32_t*)s.bgraph.p, (const int32_t*)s.bsub.p,
+              (const int64_t*)s.chunk.p, s.part.p);
+    VT_LAUNCH(ctx, "seg_sum2_final", k_seg_sum2_final, (unsigned)s.ng, kSumBlock, 0, (const double2*)s.part.p,
+              (const int64_t*)s.pbase.p, d_out);
+}
+
+void seg_seq_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b, double* d_out)
+{
+    if (s.ng == 0) return;
+    VT_LAUNCH(ctx, "seg_seq_sum2", k_seg_seq_sum2, grid_for(s.ng, 64, 4096), 64, 0, a, b, (const int64_t*)s.voff.p,
+              s.ng, d_out);
+}
+
 // ---- scans and sorts (rocPRIM) --------------------------------------------------------
 
 void exclusive_scan_i64(vt_ctx* ctx, const int64_t* in, int64_t* out, int64_t n)

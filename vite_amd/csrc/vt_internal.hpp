@@ -163,6 +163,18 @@ inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)
 // fixed tree; block partials are reduced by one block in index order.
 void tree_sum2(vt_ctx* ctx, const double* a, const double* b_sq, int64_t n, double* d_out2);
 void seq_sum2(vt_ctx* ctx, const double* a, const double* b_sq, int64_t n, double* d_out2);
+// The same sums per graph of a batch of disjoint graphs (vertex ranges voff[g]..voff[g+1]),
+// each bit-identical to tree_sum2 / seq_sum2 on that graph alone.  d_out gets 2 per graph.
+struct SegSums {
+    int32_t ng = 0;
+    int64_t nblocks = 0;
+    DBuf<int64_t> voff, chunk, pbase;
+    DBuf<int32_t> bgraph, bsub;
+    DBuf<double2> part;
+};
+void seg_sums_plan(vt_ctx* ctx, const std::vector<int64_t>& voff, SegSums& s);
+void seg_tree_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b_sq, double* d_out);
+void seg_seq_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b_sq, double* d_out);
 void exclusive_scan_i64(vt_ctx* ctx, const int64_t* in, int64_t* out, int64_t n);
 void exclusive_scan_i32(vt_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 // Stable radix sort of (key, value) pairs on key bits [0, bits).
@@ -177,5 +189,11 @@ void graph_finalize(vt_ctx* ctx, vt_graph* g);  // computes max_degree
 void graph_delete(vt_ctx* ctx, vt_graph* g);
 
 int64_t bits_for(int64_t maxval);
+
+// ---- batch of disjoint graphs (vt_louvain.hip) ----------------------------------------------
+void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,
+                 const std::vector<double>& thresh, int flags, bool use_graph, std::vector<double>& q_out,
+                 std::vector<int32_t>& iters_out, int* rounds_out);
+vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_part);
 
 }  // namespace vt

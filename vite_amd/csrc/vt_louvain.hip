@@ -119,8 +119,12 @@ struct MoveArgs {
     double* frozen;               // null when ET is off
     int64_t* target;
     double* cw;
-    const double* constant;       // device scalar 1/2m
+    const double* constant;       // 1/2m: a device scalar, or one per graph of a batch
     unsigned long long* inactive_count;
+    // Batch of disjoint graphs: graph of each vertex and whether that graph is still
+    // iterating (null for a single graph).
+    const int32_t* gid;
+    const unsigned char* gact;
 };
 
 // Inserts or finds community c; returns its slot.
@@ -153,6 +157,7 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
     using mask_t = typename MaskT<G>::type;
     const mask_t gmask = (G == 64) ? (mask_t)~0ull : (mask_t)((1ull << G) - 1);
 
+    if (a.gid && !a.gact[a.gid[v]]) return;  // its graph has left the phase
     if (a.active && !a.active[v]) {
         // Frozen vertex: clusterWeight[i] = frozenClusterWeight[i] (louvain.cpp:101-104).
         if (lane == 0) {
@@ -238,7 +243,7 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
     const double kv = a.vdeg[v];
     const double ccDeg = a.cdeg[cc];
     const int64_t ccSize = a.csize[cc];
-    const double constant = *a.constant;
+    const double constant = a.gid ? a.constant[a.gid[v]] : *a.constant;
     const double ecc = TMem<HBM>::ldd(&t.vals[cs]);
     const double eix = ecc - selfLoop;
     const double ax = ccDeg - kv;
@@ -386,28 +391,39 @@ __device__ __forceinline__ void fold_chunk(const GroupTable<G>& t, double* ecc_c
 // A vertex as the fused buckets see it: row start, degree (< 0 for a vertex early
 // termination froze), own community and weighted degree.  Rows here have degree <= 127 and
 // community ids fit 32 bits (vt_graph_upload rejects nv >= 2^31).
+// In a batch, degree -2 marks a vertex whose graph has left the phase (nothing is written)
+// and `cst` carries its graph's 1/2m.
+template <bool BATCH>
 struct VHeader {
     int32_t v;
     int32_t deg;
     int64_t e0;
     int32_t cc;
     double kv;
+    double cst;
 };
 
-__device__ __forceinline__ VHeader load_header(const MoveArgs& a, int32_t v, bool have)
+template <bool BATCH>
+__device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t v, bool have)
 {
-    VHeader h;
+    VHeader<BATCH> h;
     h.v = v;
     h.deg = -1;
     h.e0 = 0;
     h.cc = 0;
     h.kv = 0.0;
+    h.cst = 0.0;
     if (have) {
         const bool live = !(a.active && !a.active[v]);
         h.e0 = a.offs[v];
         h.deg = live ? (int32_t)(a.offs[v + 1] - h.e0) : -1;
         h.cc = (int32_t)a.curr[v];
         h.kv = a.vdeg[v];
+        if constexpr (BATCH) {
+            const int32_t gg = a.gid[v];
+            h.cst = a.constant[gg];
+            if (!a.gact[gg]) h.deg = -2;
+        }
     }
     return h;
 }
@@ -431,8 +447,8 @@ struct RowRegs {
 
 // Edge records are {int64 tail, f64 weight}; only the tail's low word and the weight are
 // read, which keeps a chunk in flight at three registers a lane.
-template <int G>
-__device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader& h, int lane, RowRegs& r)
+template <int G, bool BATCH>
+__device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader<BATCH>& h, int lane, RowRegs& r)
 {
     const int64_t j0 = h.e0 + lane, j1 = h.e0 + G + lane;
     const int32_t* words = reinterpret_cast<const int32_t*>(a.edges);
@@ -462,20 +478,20 @@ __device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, 
     r.self1 = r.ok1 && r.t1 == v;
 }
 
-template <int G, int CAP, int BLOCK, bool PIPE>
+template <int G, int CAP, int BLOCK, bool PIPE, bool BATCH>
 __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const GroupTable<G>& t, double* ecc_cell,
                                                   const int32_t* __restrict__ list, int64_t n, int64_t idx,
                                                   int64_t stride, int lane, int gbase, unsigned* inactive_local)
 {
     static_assert(CAP == 4 * G && CAP <= 256, "occ holds 8-bit slot indexes");
     if (idx >= n) return;
-    VHeader h = load_header(a, list[idx], true);
+    VHeader<BATCH> h = load_header<BATCH>(a, list[idx], true);
     int32_t vn = (idx + stride < n) ? list[idx + stride] : 0;
     RowRegs r;
     if constexpr (PIPE) row_issue_edges<G>(a, h, lane, r);
-    VHeader hn = load_header(a, vn, idx + stride < n);
+    VHeader<BATCH> hn = load_header<BATCH>(a, vn, idx + stride < n);
     if constexpr (PIPE) row_issue_gathers(a, h.v, r);
-    const double constant = *a.constant;
+    const double constant0 = BATCH ? 0.0 : *a.constant;
     while (idx < n) {
         const bool has_next = PIPE && idx + stride < n;
         // list entry two ahead, header one ahead (and with PIPE the next vertex's edges)
@@ -488,11 +504,14 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             row_issue_edges<G>(a, h, lane, r);
             row_issue_gathers(a, h.v, r);
         }
-        const VHeader hnn = load_header(a, vnn, i2 < n);
+        const VHeader<BATCH> hnn = load_header<BATCH>(a, vnn, i2 < n);
+        const double constant = BATCH ? h.cst : constant0;
 
         const int32_t v = h.v;
         const int64_t cc = h.cc;
-        if (h.deg < 0) {
+        if (h.deg < -1) {
+            // a batch graph that has left the phase: nothing to do
+        } else if (h.deg < 0) {
             if (lane == 0) {
                 a.cw[v] = a.frozen[v];
                 atomicAdd(inactive_local, 1u);
@@ -602,7 +621,7 @@ struct GroupLds {
 
 // Runs one degree bucket inside a block: block `blk` of `nblk` owns list entries
 // blk*NG + g, stepping by nblk*NG.  Every thread of the block must call it.
-template <int G, int CAP, int BLOCK, bool TWO>
+template <int G, int CAP, int BLOCK, bool TWO, bool BATCH = false>
 __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const int32_t* __restrict__ list, int64_t n,
                                            int64_t blk, int64_t nblk, unsigned* inactive)
 {
@@ -629,7 +648,7 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
         GroupTable<G> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, (uint8_t*)(lds + L::occ) + g * 2 * G,
                         wst + g * (G + 1) + 1, (uint32_t)(CAP - 1)};
         double* ecc_cell = (double*)(lds + L::ecc) + g;
-        run_two_pipelined<G, CAP, BLOCK, kPipeRows>(a, t, ecc_cell, list, n, idx, stride, lane, gbase, inactive);
+        run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH>(a, t, ecc_cell, list, n, idx, stride, lane, gbase, inactive);
     } else {
         using L = LdsLayout<G, CAP, BLOCK>;
         using mask_t = typename L::mask_t;
@@ -673,6 +692,7 @@ struct SweepPlan {
 #ifndef VT_FUSED_WPE
 #define VT_FUSED_WPE 1
 #endif
+template <bool BATCH>
 __global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(VT_FUSED_WPE)))
 k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
 {
@@ -684,22 +704,23 @@ k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
     const int64_t blk = blockIdx.x - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
     const int32_t* list = lists + plan.off[b];
     switch (b) {
-        case 0: run_bucket<4, 16, 256, true>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
-        case 1: run_bucket<8, 32, 256, true>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
-        case 2: run_bucket<16, 64, 256, true>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
-        default: run_bucket<32, 128, 256, true>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+        case 0: run_bucket<4, 16, 256, true, BATCH>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
+        case 1: run_bucket<8, 32, 256, true, BATCH>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+        case 2: run_bucket<16, 64, 256, true, BATCH>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
+        default: run_bucket<32, 128, 256, true, BATCH>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
     }
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
 
 // Bucket 4 (64 <= degree <= 127): four 64-lane groups per block, 256-slot tables.
+template <bool BATCH>
 __global__ void __launch_bounds__(256) k_local_move_mid(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
 {
     __shared__ __align__(16) char lds[kMidLds];
     __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
-    run_bucket<64, 256, 256, true>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
+    run_bucket<64, 256, 256, true, BATCH>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
@@ -1012,11 +1033,11 @@ __global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, cons
 // (louvain.cpp:2534-2547).
 __global__ void k_fold_events(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
                               const int64_t* __restrict__ target, const double* __restrict__ vdeg,
-                              double* __restrict__ cdeg, int64_t* __restrict__ csize)
+                              double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none)
 {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
-        if (j > 0 && keys[j - 1] == c) continue;
+        if (c == none || (j > 0 && keys[j - 1] == c)) continue;
         double acc = 0.0;
         int64_t cnt = 0;
         for (int64_t k = j; k < n && keys[k] == c; k++) {
@@ -1071,6 +1092,70 @@ __global__ void k_rotate_et2(int64_t nv, int num_iters, double one_minus_delta, 
     }
 }
 
+// ---- batch of disjoint graphs ------------------------------------------------------------
+
+__global__ void k_seg_constant(const double* __restrict__ sums, int32_t ng, double* __restrict__ constant)
+{
+    for (int32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x)
+        constant[g] = 1.0 / sums[2 * g];
+}
+
+// Two events per vertex at fixed positions (arrival at target, departure from curr); a
+// vertex that does not move, or whose graph has left the phase, gets key `none`, which sorts
+// after every community.  A stable sort then gives each community its events in vertex
+// order, as the one-graph path does with a compacted mover list.
+__global__ void k_batch_events(int64_t nv, const int64_t* __restrict__ target, const int64_t* __restrict__ curr,
+                               const int32_t* __restrict__ gid, const unsigned char* __restrict__ gact, uint32_t none,
+                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+        const bool mv = gact[gid[v]] && target[v] != curr[v];
+        keys[2 * v] = mv ? (uint32_t)target[v] : none;
+        vals[2 * v] = (uint32_t)v;
+        keys[2 * v + 1] = mv ? (uint32_t)curr[v] : none;
+        vals[2 * v + 1] = (uint32_t)v;
+    }
+}
+
+// The end of the reference loop body per graph (louvain.cpp:540-577): currMod from the
+// sums, leave when it gained less than thresh, else prevMod = max(currMod, lower) and rotate.
+__global__ void k_batch_decide(int32_t ng, const double* __restrict__ sums, const double* __restrict__ constant,
+                               const double* __restrict__ thresh, const double* __restrict__ lower,
+                               unsigned char* __restrict__ gact, unsigned char* __restrict__ grot,
+                               double* __restrict__ prev, int32_t* __restrict__ iters, int32_t max_iters,
+                               int32_t* __restrict__ nactive)
+{
+    for (int32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+        grot[g] = 0;
+        if (!gact[g]) continue;
+        iters[g] += 1;
+        const double c = constant[g];
+        const double q = fabs((sums[2 * g] * c) - (sums[2 * g + 1] * c * c));
+        if ((q - prev[g]) < thresh[g]) {
+            gact[g] = 0;
+            continue;
+        }
+        prev[g] = q < lower[g] ? lower[g] : q;
+        if (max_iters > 0 && iters[g] >= max_iters) {
+            gact[g] = 0;
+            continue;
+        }
+        grot[g] = 1;
+        atomicAdd(nactive, 1);
+    }
+}
+
+// past <- curr <- target for the graphs that continue (louvain.cpp:565-570).
+__global__ void k_batch_rotate(int64_t nv, const int32_t* __restrict__ gid, const unsigned char* __restrict__ grot,
+                               int64_t* __restrict__ past, int64_t* __restrict__ curr, const int64_t* __restrict__ target)
+{
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+        if (!grot[gid[v]]) continue;
+        past[v] = curr[v];
+        curr[v] = target[v];
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Host side
 // ------------------------------------------------------------------------------------------
@@ -1109,6 +1194,11 @@ struct Phase {
     std::vector<int64_t> color_ptr;      // colorPtr (louvain.cpp:647-661)
     int64_t hub_cap = 0;
     int hub_blocks = 0;
+    // Batch of disjoint graphs (null for one graph): graph of each vertex, per-graph
+    // "still iterating" flags and per-graph 1/2m.
+    const int32_t* gid = nullptr;
+    const unsigned char* gact = nullptr;
+    double* gconst = nullptr;
 
     Phase(vt_ctx* c, const vt_graph* gr) : ctx(c), g(gr), nv(gr->nv) {}
 
@@ -1163,7 +1253,7 @@ struct Phase {
         std::vector<int64_t> h(nkeys, 0);
         if (nv) {
             VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv,
-                      ncol > 1 ? d_colors : nullptr, num_color, lpv_max_degree(), k0, (uint32_t*)lists.p);
+                      ncol > 1 ? d_colors : nullptr, num_color, gid ? 0 : lpv_max_degree(), k0, (uint32_t*)lists.p);
             dzero(ctx, hist, 8 * nkeys);
             key_histogram(k0, nv, hist, (int)nkeys);
             bool alt = false;
@@ -1257,8 +1347,10 @@ struct Phase {
         a.frozen = et ? frozen.p : nullptr;
         a.target = target.p;
         a.cw = cw.p;
-        a.constant = constant();
+        a.constant = gid ? gconst : constant();
         a.inactive_count = ucount.p;
+        a.gid = gid;
+        a.gact = gact;
         return a;
     }
 
@@ -1272,12 +1364,17 @@ struct Phase {
             VT_LAUNCH(ctx, "local_move_lpv", kern, (unsigned)nblk, 64, 0, a, (const int32_t*)(lists.p + cs.lpv_off), cs.lpv_n,
                       chunk);
         }
-        if (cs.fused_blocks)
-            VT_LAUNCH(ctx, "local_move", k_local_move, (unsigned)cs.fused_blocks, kFusedBlock, 0, a,
-                      (const int32_t*)lists.p, cs.plan);
-        if (cs.mid_n)
-            VT_LAUNCH(ctx, "local_move_mid", k_local_move_mid, (unsigned)std::min<int64_t>((cs.mid_n + 3) / 4, 16384), 256,
-                      0, a, (const int32_t*)(lists.p + cs.mid_off), cs.mid_n);
+        const bool batch = a.gid != nullptr;
+        if (cs.fused_blocks) {
+            auto kern = batch ? k_local_move<true> : k_local_move<false>;
+            VT_LAUNCH(ctx, "local_move", kern, (unsigned)cs.fused_blocks, kFusedBlock, 0, a, (const int32_t*)lists.p,
+                      cs.plan);
+        }
+        if (cs.mid_n) {
+            auto kern = batch ? k_local_move_mid<true> : k_local_move_mid<false>;
+            VT_LAUNCH(ctx, "local_move_mid", kern, (unsigned)std::min<int64_t>((cs.mid_n + 3) / 4, 16384), 256, 0, a,
+                      (const int32_t*)(lists.p + cs.mid_off), cs.mid_n);
+        }
         if (cs.wide_n)
             VT_LAUNCH(ctx, "local_move_wide", k_local_move_wide, (unsigned)std::min<int64_t>(cs.wide_n, 8192), 64, 0, a,
                       (const int32_t*)(lists.p + cs.wide_off), cs.wide_n);
@@ -1314,7 +1411,7 @@ struct Phase {
         const uint32_t* ks = alt ? k1 : k0;
         const uint32_t* vs = alt ? v1 : v0;
         VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, target.p, vdeg.p,
-                  dst_cdeg, dst_csize);
+                  dst_cdeg, dst_csize, 0xFFFFFFFFu);
         return movers;
     }
 
@@ -1470,6 +1567,137 @@ double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cv
         delete st;
         throw;
     }
+}
+
+// One Louvain phase (plain mode) of every graph of a batch at once: all graphs iterate in
+// lockstep, a graph that stops improving leaves the loop and stays frozen while the others
+// go on.  Each graph sees exactly the arithmetic it would see alone (the local move reads
+// only its own vertices and communities and its own 1/2m; the delta folds are per
+// community; sums use the per-graph trees of seg_tree_sum2), so per graph the return value,
+// iteration count and partition equal those of run_phase on that graph alone.  The
+// partition is left on `u` (u->part) for the rebuild.
+//
+// With `use_graph`, the iteration after the first is replayed from two captured HIP graphs
+// (local move + events; folds + sums + decisions + rotation) around the event sort, which
+// stays outside capture because the library sort clears its buffers with memsets.
+void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,
+                 const std::vector<double>& thresh, int flags, bool use_graph, std::vector<double>& q_out,
+                 std::vector<int32_t>& iters_out, int* rounds_out)
+{
+    const int32_t ng = seg.ng;
+    const int64_t nv = u->nv;
+    Phase ph(ctx, u);
+    ph.alloc(false);
+    DBuf<double> gconst(ctx, ng), gsums(ctx, 2 * (size_t)ng), gprev(ctx, ng), gthr(ctx, ng), glow(ctx, ng);
+    DBuf<unsigned char> gact(ctx, ng), grot(ctx, ng);
+    DBuf<int32_t> giters(ctx, ng), nact(ctx, 1);
+    ph.gid = d_gid;
+    ph.gact = gact.p;
+    ph.gconst = gconst.p;
+    const bool seq = (flags & VT_FLAG_SEQUENTIAL_SUMS) != 0;
+
+    VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, u->offs, u->edges, nv, ph.vdeg.p, ph.cdeg.p,
+              ph.csize.p, ph.past.p, ph.curr.p, ph.target.p, nullptr, nullptr, nullptr, nullptr);
+    if (seq) seg_seq_sum2(ctx, seg, ph.vdeg.p, nullptr, gsums.p);
+    else seg_tree_sum2(ctx, seg, ph.vdeg.p, nullptr, gsums.p);
+    VT_LAUNCH(ctx, "seg_constant", k_seg_constant, grid_for(ng, 256, 64), 256, 0, (const double*)gsums.p, ng, gconst.p);
+    h2d(ctx, gthr.p, thresh.data(), 8 * (size_t)ng);
+    h2d(ctx, glow.p, lower.data(), 8 * (size_t)ng);
+    d2d(ctx, gprev.p, glow.p, 8 * (size_t)ng);
+    dzero(ctx, giters.p, 4 * (size_t)ng);
+    {
+        std::vector<unsigned char> ones((size_t)ng, 1);
+        h2d(ctx, gact.p, ones.data(), (size_t)ng);
+    }
+    ph.build_lists(nullptr, 1, false);
+
+    const int64_t nev = 2 * std::max<int64_t>(nv, 1);
+    uint32_t* k0 = (uint32_t*)scratch(ctx, "bev_k0", 4 * (size_t)nev);
+    uint32_t* v0 = (uint32_t*)scratch(ctx, "bev_v0", 4 * (size_t)nev);
+    uint32_t* k1 = (uint32_t*)scratch(ctx, "bev_k1", 4 * (size_t)nev);
+    uint32_t* v1 = (uint32_t*)scratch(ctx, "bev_v1", 4 * (size_t)nev);
+    const uint32_t none = (uint32_t)nv;
+    const int bits = (int)bits_for(std::max<int64_t>(nv, 1));
+    bool alt = false;
+
+    auto pre = [&]() {
+        ph.sweep(ph.sweeps[0]);
+        VT_LAUNCH(ctx, "batch_events", k_batch_events, grid_for(nv, 256, 8192), 256, 0, nv, (const int64_t*)ph.target.p,
+                  (const int64_t*)ph.curr.p, d_gid, (const unsigned char*)gact.p, none, k0, v0);
+    };
+    auto sort = [&]() { sort_pairs_u32(ctx, k0, v0, 2 * nv, bits, k1, v1, &alt); };
+    auto post = [&]() {
+        const uint32_t* ks = alt ? k1 : k0;
+        const uint32_t* vs = alt ? v1 : v0;
+        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(2 * nv, 256, 16384), 256, 0, ks, vs, 2 * nv,
+                  (const int64_t*)ph.target.p, (const double*)ph.vdeg.p, ph.cdeg.p, ph.csize.p, none);
+        if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
+        else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
+        dzero(ctx, nact.p, 4);
+        VT_LAUNCH(ctx, "batch_decide", k_batch_decide, grid_for(ng, 256, 64), 256, 0, ng, (const double*)gsums.p,
+                  (const double*)gconst.p, (const double*)gthr.p, (const double*)glow.p, gact.p, grot.p, gprev.p,
+                  giters.p, 0, nact.p);
+        VT_LAUNCH(ctx, "batch_rotate", k_batch_rotate, grid_for(nv, 256, 8192), 256, 0, nv, d_gid,
+                  (const unsigned char*)grot.p, ph.past.p, ph.curr.p, (const int64_t*)ph.target.p);
+    };
+
+    int rounds = 0;
+    int32_t active = ng;
+    hipGraphExec_t gpre = nullptr, gpost = nullptr;
+    auto capture = [&](auto&& body, hipGraphExec_t* exec) {
+        const bool prof = ctx->profiling;
+        ctx->profiling = false;  // no timing events inside a capture
+        hipGraph_t graph = nullptr;
+        VT_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            ctx->profiling = prof;
+            throw;
+        }
+        VT_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
+        ctx->profiling = prof;
+        const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        VT_HIP(ctx, e);
+    };
+    auto destroy = [&]() {
+        if (gpre) (void)hipGraphExecDestroy(gpre);
+        if (gpost) (void)hipGraphExecDestroy(gpost);
+        gpre = gpost = nullptr;
+    };
+    try {
+        while (active > 0) {
+            if (use_graph && rounds == 1) {
+                capture(pre, &gpre);
+                capture(post, &gpost);
+            }
+            if (gpre) {
+                VT_HIP(ctx, hipGraphLaunch(gpre, ctx->stream));
+                sort();
+                VT_HIP(ctx, hipGraphLaunch(gpost, ctx->stream));
+            } else {
+                pre();
+                sort();
+                post();
+            }
+            rounds++;
+            d2h(ctx, &active, nact.p, 4);
+        }
+    } catch (...) {
+        destroy();
+        throw;
+    }
+    destroy();
+    if (!u->part) u->part = (int64_t*)dalloc(ctx, 8 * (size_t)std::max<int64_t>(nv, 1));
+    d2d(ctx, u->part, ph.past.p, 8 * (size_t)nv);
+    q_out.assign((size_t)ng, 0.0);
+    iters_out.assign((size_t)ng, 0);
+    d2h(ctx, q_out.data(), gprev.p, 8 * (size_t)ng);
+    d2h(ctx, iters_out.data(), giters.p, 4 * (size_t)ng);
+    if (rounds_out) *rounds_out = rounds;
 }
 
 }  // namespace vt
