@@ -105,6 +105,47 @@ def summarize_profile(d):
     return "\n".join(lines) + "\n"
 
 
+def batched_form(ctx, args):
+    """The batched form: args.batch_graphs seeded RGGs, n log-uniform in [2^8, 2^16], every
+    other graph on average with 1% extra edges; all run through the threshold-cycling loop
+    at once (vt_batch_run), then a strided sample of them one graph at a time (vt_run), whose
+    results must be identical."""
+    import numpy as np
+    import vite_amd
+
+    rng = np.random.default_rng(args.seed)
+    count = args.batch_graphs
+    ns = np.rint(2.0 ** rng.uniform(8.0, 16.0, size=count)).astype(np.int64)
+    extras = np.where(rng.random(count) < 0.5, 1.0, 0.0)
+    t0 = time.perf_counter()
+    graphs = [ctx.gen_rgg(args.seed * 1000003 + k, int(ns[k]), float(extras[k])) for k in range(count)]
+    gen_s = time.perf_counter() - t0
+    b = vite_amd.Batch(ctx, graphs)
+    log(f"batch: {count} graphs, {b.total_nv} vertices, {b.total_ne} directed entries (generated in {gen_s:.1f}s)")
+    kw = dict(threshold_cycling=True)
+    t0 = time.perf_counter()
+    res, st = b.run(**kw)
+    batch_s = time.perf_counter() - t0
+    log(f"batch: {batch_s:.3f}s, {count / batch_s:.1f} graphs/s, {st['phases']} phases, {st['rounds']} rounds")
+    stride = max(1, count // max(1, args.batch_sample))
+    sample = list(range(0, count, stride))[:args.batch_sample]
+    t0 = time.perf_counter()
+    alone = [graphs[k].run(**kw) for k in sample]
+    plain_s = time.perf_counter() - t0
+    same = all(a.call_q == res[k].call_q and a.call_iters == res[k].call_iters and a.final_q == res[k].final_q
+               for a, k in zip(alone, sample))
+    plain_gps = len(sample) / plain_s
+    log(f"one graph per launch: {len(sample)} graphs in {plain_s:.3f}s, {plain_gps:.1f} graphs/s; identical={same}")
+    b.free()
+    for g in graphs:
+        g.free()
+    return {"graphs": count, "graphs_per_s": count / batch_s, "seconds": batch_s,
+            "one_per_launch_graphs_per_s": plain_gps, "one_per_launch_sample": len(sample),
+            "speedup": (count / batch_s) / plain_gps, "identical_per_graph_results": same,
+            "phases": st["phases"], "rounds": st["rounds"], "total_nv": b.total_nv, "total_ne": b.total_ne,
+            "workload": "RGG n log-uniform in [2^8, 2^16], 1% extra edges on half, threshold cycling"}
+
+
 def measured_traffic(kernel, workload_tag):
     """HBM bytes per launch of `kernel` from the committed rocprof counter summary of this
     workload (FETCH_SIZE x FETCH_SCALE + WRITE_SIZE x WRITE_SCALE), or None."""
@@ -137,6 +178,9 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttq", action="store_true", help="skip the time-to-final-Q run")
+    ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batched-form run (0 = skip)")
+    ap.add_argument("--batch-sample", type=int, default=256,
+                    help="graphs of the batch also run one graph per launch (the comparison)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -224,6 +268,9 @@ def main():
                                   "calls": len(r.call_q), "total_iters": r.total_iters,
                                   "clustering_seconds": r.clustering_seconds, "flags": "-i"}
         log(f"time to final Q: {ttq:.2f}s q={r.final_q:.9f} levels={len(r.level_q)} iters={r.total_iters}")
+
+    if args.batch_graphs > 0:
+        out["batched"] = batched_form(ctx, args)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -393,13 +393,17 @@ __device__ __forceinline__ void fold_chunk(const GroupTable<G>& t, double* ecc_c
 // community ids fit 32 bits (vt_graph_upload rejects nv >= 2^31).
 // In a batch, degree -2 marks a vertex whose graph has left the phase (nothing is written)
 // and `cst` carries its graph's 1/2m.
-template <bool BATCH>
-struct VHeader {
+struct VHeaderBase {
     int32_t v;
     int32_t deg;
     int64_t e0;
     int32_t cc;
     double kv;
+};
+template <bool BATCH>
+struct VHeader : VHeaderBase {};
+template <>
+struct VHeader<true> : VHeaderBase {
     double cst;
 };
 
@@ -412,7 +416,7 @@ __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t
     h.e0 = 0;
     h.cc = 0;
     h.kv = 0.0;
-    h.cst = 0.0;
+    if constexpr (BATCH) h.cst = 0.0;
     if (have) {
         const bool live = !(a.active && !a.active[v]);
         h.e0 = a.offs[v];
@@ -429,7 +433,7 @@ __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t
 }
 
 #ifndef VT_PIPE_ROWS
-#define VT_PIPE_ROWS 0
+#define VT_PIPE_ROWS 1
 #endif
 constexpr bool kPipeRows = VT_PIPE_ROWS;
 
@@ -505,11 +509,12 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             row_issue_gathers(a, h.v, r);
         }
         const VHeader<BATCH> hnn = load_header<BATCH>(a, vnn, i2 < n);
-        const double constant = BATCH ? h.cst : constant0;
+        double constant = constant0;
+        if constexpr (BATCH) constant = h.cst;
 
         const int32_t v = h.v;
         const int64_t cc = h.cc;
-        if (h.deg < -1) {
+        if (BATCH && h.deg < -1) {
             // a batch graph that has left the phase: nothing to do
         } else if (h.deg < 0) {
             if (lane == 0) {
@@ -690,7 +695,7 @@ struct SweepPlan {
 };
 
 #ifndef VT_FUSED_WPE
-#define VT_FUSED_WPE 1
+#define VT_FUSED_WPE 7
 #endif
 template <bool BATCH>
 __global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(VT_FUSED_WPE)))
