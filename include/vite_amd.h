@@ -62,6 +62,23 @@ int vt_ctx_kernel_stats(const vt_ctx* ctx, const char* name, double* avg_ms, int
                         double* total_ms);
 int vt_ctx_reset_stats(vt_ctx* ctx);
 
+/* Launch shape of the local move.  Rows of degree <= bucket_max_degree[b] (and above the
+ * previous bound) run with 4, 8, 16, 32 or 64 lanes per vertex (b = 0..4; each bound at most
+ * 2 x lanes - 1) or one 64-lane wave with a 2048-slot table (b = 5, at most 1023); longer rows
+ * use tables in HBM.  blocks_per_bucket caps the workgroups one bucket gets per launch.
+ * Any valid setting gives bit-identical results; only the speed changes.
+ * Default {7, 15, 31, 63, 127, 1023}, 16384. */
+typedef struct {
+    int32_t bucket_max_degree[6];
+    int32_t blocks_per_bucket;
+} vt_tuning;
+int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out);
+int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t);
+/* Autotuner: from the state the first `prep` iterations of a phase-0 run (threshold 1e-3)
+ * reach on g, times `reps` local-move sweeps under candidate bucket bounds and block caps
+ * (coordinate descent from the current setting), keeps the fastest on ctx and returns it. */
+int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning* out);
+
 /* ---- graphs ------------------------------------------------------------------- */
 /* Copies a host CSR (offsets[nv+1], edges[ne]) to the device.  Rejects offsets that are
  * not monotone from 0 to ne and tails outside [0, nv). */

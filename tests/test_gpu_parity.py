@@ -211,3 +211,55 @@ def test_phase_sweep_state_replay(ctx, name, prep):
     assert np.array_equal(s1["target"], t_o)
     assert np.array_equal(s1["cw"].view(np.uint64), cw_o.view(np.uint64))
     st.end(want_cvect=False)
+
+
+TUNINGS = [dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=16384),
+           dict(bucket_max_degree=[5, 11, 23, 47, 95, 300], blocks_per_bucket=8),
+           dict(bucket_max_degree=[-1, -1, -1, -1, 127, 1023], blocks_per_bucket=64),
+           dict(bucket_max_degree=[7, 15, 31, 63, 127, 130], blocks_per_bucket=4096)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=[str(i) for i in range(len(TUNINGS))])
+@pytest.mark.parametrize("name", ["rgg", "ba", "hub", "dense", "multi"])
+def test_tuning_does_not_change_results(ctx, name, tuning):
+    """Bucket bounds and block caps only change the launch shape: sweeps and phases stay
+    bit-identical to the oracle under every valid setting."""
+    g, dg = _upload(ctx, name)
+    default = ctx.tuning
+    try:
+        ctx.set_tuning(**tuning)
+        curr, csize, cdeg, vdeg, c = graphs.random_state(g, 7)
+        t_o, cw_o, ds_o, dd_o = po.sweep(g, curr, csize, cdeg, vdeg, c)
+        t, cw, ds, dd = dg.sweep(curr, csize, cdeg, vdeg, c)
+        assert np.array_equal(t, t_o)
+        assert np.array_equal(cw.view(np.uint64), cw_o.view(np.uint64))
+        q_o, cv_o, it_o = po.louvain_phase(g)
+        q, cv, it = dg.louvain_phase(sequential=True)
+        assert (q, it) == (q_o, it_o)
+        assert np.array_equal(cv, cv_o)
+    finally:
+        ctx.set_tuning(**default)
+
+
+def test_tuning_rejects_bounds_beyond_kernel_capacity(ctx):
+    default = ctx.tuning
+    for bad in [[8, 15, 31, 63, 127, 1023], [7, 15, 31, 64, 127, 1023], [7, 15, 31, 63, 127, 1024]]:
+        with pytest.raises(vite_amd.VtError):
+            ctx.set_tuning(bucket_max_degree=bad)
+    with pytest.raises(vite_amd.VtError):
+        ctx.set_tuning(blocks_per_bucket=4)
+    assert ctx.tuning == default
+
+
+def test_autotune_keeps_results(ctx):
+    g, dg = _upload(ctx, "dense")
+    default = ctx.tuning
+    try:
+        t = ctx.autotune(dg, prep=1, reps=1)
+        assert t == ctx.tuning
+        q_o, cv_o, it_o = po.louvain_phase(g)
+        q, cv, it = dg.louvain_phase(sequential=True)
+        assert (q, it) == (q_o, it_o)
+        assert np.array_equal(cv, cv_o)
+    finally:
+        ctx.set_tuning(**default)

@@ -67,6 +67,10 @@ class _RunResult(C.Structure):
 
 
 # Every symbol include/vite_amd.h declares, with its ctypes signature.
+class _Tuning(C.Structure):
+    _fields_ = [("bucket_max_degree", C.c_int32 * 6), ("blocks_per_bucket", C.c_int32)]
+
+
 class _BatchStats(C.Structure):
     _fields_ = [("phases", C.c_int32), ("rounds", C.c_int32), ("seconds", C.c_double)]
 
@@ -106,6 +110,9 @@ _SIGS = {
     "vt_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_RunParams), C.POINTER(_RunResult),
                                C.POINTER(_BatchStats)]),
     "vt_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vt_ctx_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(_Tuning)]),
+    "vt_ctx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(_Tuning)]),
+    "vt_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(_Tuning)]),
 }
 
 
@@ -188,6 +195,23 @@ class Context:
     @property
     def bytes_in_use(self) -> int:
         return int(_lib.vt_ctx_bytes_in_use(self.h))
+
+    @property
+    def tuning(self) -> dict:
+        t = _Tuning()
+        self.check(_lib.vt_ctx_get_tuning(self.h, C.byref(t)))
+        return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket)
+
+    def set_tuning(self, bucket_max_degree=None, blocks_per_bucket=None):
+        cur = self.tuning
+        t = _Tuning((C.c_int32 * 6)(*(bucket_max_degree or cur["bucket_max_degree"])),
+                    blocks_per_bucket or cur["blocks_per_bucket"])
+        self.check(_lib.vt_ctx_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, g: "DeviceGraph", prep: int = 2, reps: int = 3) -> dict:
+        t = _Tuning()
+        self.check(_lib.vt_autotune(self.h, g.h, prep, reps, C.byref(t)))
+        return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket)
 
     def set_profiling(self, on: bool):
         self.check(_lib.vt_ctx_set_profiling(self.h, int(on)))

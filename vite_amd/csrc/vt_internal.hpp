@@ -56,6 +56,8 @@ struct vt_ctx {
     std::vector<int64_t> trace_moves;
     // pinned host staging for small readbacks
     void* pinned = nullptr;
+    // sweep launch shape (performance only: every setting gives the same results)
+    vt_tuning tuning = {{7, 15, 31, 63, 127, 1023}, 16384};
 };
 
 struct vt_graph {

@@ -54,9 +54,17 @@ constexpr int kLpvMaxDeg = 64;     // the largest row k_local_move_lpv is instan
 constexpr int kLpvArena = 2560;
 constexpr int kHubBucket = 6;
 
-__host__ __device__ inline int degree_bucket(int64_t d)
+// Bucket of a row of degree d under per-bucket maximum degrees mx[0..5] (the tuning; the
+// default is 7, 15, 31, 63, 127, 1023): rows above mx[5] take the HBM-table kernel.
+struct BucketBounds {
+    int32_t mx[6];
+};
+
+__host__ __device__ inline int degree_bucket(int64_t d, const BucketBounds& bb)
 {
-    return d <= 7 ? 0 : d <= 15 ? 1 : d <= 31 ? 2 : d <= 63 ? 3 : d <= 127 ? 4 : d <= 1023 ? 5 : 6;
+    for (int b = 0; b < 6; b++)
+        if (d <= bb.mx[b]) return b;
+    return 6;
 }
 
 template <int G> struct MaskT { using type = uint32_t; };
@@ -610,6 +618,12 @@ struct LdsLayout {
     static constexpr size_t bytes = align16(ecc + sizeof(double) * NG);
 };
 
+#ifndef VT_XCD_RANGES
+#define VT_XCD_RANGES 1
+#endif
+constexpr bool kXcdRanges = VT_XCD_RANGES;
+constexpr int kXcds = 8;
+
 // LDS carve-out of the two-chunk group buckets.
 template <int G, int CAP, int BLOCK>
 struct GroupLds {
@@ -653,7 +667,19 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
         GroupTable<G> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, (uint8_t*)(lds + L::occ) + g * 2 * G,
                         wst + g * (G + 1) + 1, (uint32_t)(CAP - 1)};
         double* ecc_cell = (double*)(lds + L::ecc) + g;
-        run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH>(a, t, ecc_cell, list, n, idx, stride, lane, gbase, inactive);
+        if (kXcdRanges && nblk % kXcds == 0) {
+            // Workgroups are dealt to the XCDs round-robin, so block blk runs on XCD
+            // blk % 8 (block counts and bucket starts are multiples of 8).  Give each XCD a
+            // contiguous eighth of the list: neighbouring vertices then share one L2.
+            const int64_t x = blk % kXcds, per = (n + kXcds - 1) / kXcds;
+            const int64_t lo = x * per < n ? x * per : n, hi = lo + per < n ? lo + per : n;
+            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH>(a, t, ecc_cell, list + lo, hi - lo,
+                                                               (blk / kXcds) * (BLOCK / G) + g,
+                                                               (nblk / kXcds) * (BLOCK / G), lane, gbase, inactive);
+        } else {
+            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH>(a, t, ecc_cell, list, n, idx, stride, lane, gbase,
+                                                               inactive);
+        }
     } else {
         using L = LdsLayout<G, CAP, BLOCK>;
         using mask_t = typename L::mask_t;
@@ -964,11 +990,12 @@ __global__ void k_constant(const double* __restrict__ sums, double* __restrict__
 
 // Sort keys for the sweep lists: (color bucket, degree bucket).
 __global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const int64_t* __restrict__ colors,
-                            int64_t num_color, int lpv_max, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                            int64_t num_color, int lpv_max, BucketBounds bb, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t d = offs[i + 1] - offs[i];
-        const uint32_t b = (uint32_t)(d <= lpv_max ? kLpvBucket : degree_bucket(d));
+        const uint32_t b = (uint32_t)(d <= lpv_max ? kLpvBucket : degree_bucket(d, bb));
         uint32_t c = 0;
         if (colors) c = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
         keys[i] = c * kNumBuckets + b;
@@ -1207,6 +1234,13 @@ struct Phase {
 
     Phase(vt_ctx* c, const vt_graph* gr) : ctx(c), g(gr), nv(gr->nv) {}
 
+    BucketBounds bounds() const
+    {
+        BucketBounds bb;
+        for (int b = 0; b < 6; b++) bb.mx[b] = ctx->tuning.bucket_max_degree[b];
+        return bb;
+    }
+
     double* sums() { return scal.p; }
     double* constant() { return scal.p + 2; }
     double* q() { return scal.p + 3; }
@@ -1258,7 +1292,8 @@ struct Phase {
         std::vector<int64_t> h(nkeys, 0);
         if (nv) {
             VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv,
-                      ncol > 1 ? d_colors : nullptr, num_color, gid ? 0 : lpv_max_degree(), k0, (uint32_t*)lists.p);
+                      ncol > 1 ? d_colors : nullptr, num_color, gid ? 0 : lpv_max_degree(), bounds(), k0,
+                      (uint32_t*)lists.p);
             dzero(ctx, hist, 8 * nkeys);
             key_histogram(k0, nv, hist, (int)nkeys);
             bool alt = false;
@@ -1278,7 +1313,9 @@ struct Phase {
                     cs.plan.bbeg[b] = blocks;
                     cs.plan.off[b] = off;
                     cs.plan.n[b] = cnt;
-                    blocks += std::min<int64_t>((cnt + kGroups[b] - 1) / kGroups[b], 16384);
+                    const int64_t nb = std::min<int64_t>((cnt + kGroups[b] - 1) / kGroups[b],
+                                                         ctx->tuning.blocks_per_bucket);
+                    blocks += (nb + kXcds - 1) / kXcds * kXcds;
                 } else if (b == kFusedBuckets) {
                     cs.mid_off = off;
                     cs.mid_n = cnt;
@@ -1377,7 +1414,8 @@ struct Phase {
         }
         if (cs.mid_n) {
             auto kern = batch ? k_local_move_mid<true> : k_local_move_mid<false>;
-            VT_LAUNCH(ctx, "local_move_mid", kern, (unsigned)std::min<int64_t>((cs.mid_n + 3) / 4, 16384), 256, 0, a,
+            const int64_t nb = std::min<int64_t>((cs.mid_n + 3) / 4, ctx->tuning.blocks_per_bucket);
+            VT_LAUNCH(ctx, "local_move_mid", kern, (unsigned)((nb + kXcds - 1) / kXcds * kXcds), 256, 0, a,
                       (const int32_t*)(lists.p + cs.mid_off), cs.mid_n);
         }
         if (cs.wide_n)
@@ -1754,6 +1792,109 @@ int vt_phase_sweep(vt_ctx* ctx, vt_phase* st)
         VT_HIP(ctx, hipSetDevice(ctx->device));
         for (const auto& cs : st->ph.sweeps) st->ph.sweep(cs);
         sync(ctx);
+    });
+}
+
+static void check_tuning(vt_ctx* ctx, const vt_tuning* t)
+{
+    // a row in bucket b has degree <= bound[b]; the kernel of bucket b handles up to cap[b]
+    static const int32_t cap[6] = {7, 15, 31, 63, 127, 1023};
+    for (int b = 0; b < 6; b++)
+        if (t->bucket_max_degree[b] < -1 || t->bucket_max_degree[b] > cap[b])
+            fail(ctx, VT_ERR_ARG, "vt_tuning: bucket_max_degree[%d] = %d outside [-1, %d]", b, t->bucket_max_degree[b],
+                 cap[b]);
+    if (t->blocks_per_bucket < 8 || t->blocks_per_bucket > (1 << 20))
+        fail(ctx, VT_ERR_ARG, "vt_tuning: blocks_per_bucket = %d outside [8, 2^20]", t->blocks_per_bucket);
+}
+
+int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out)
+{
+    if (!ctx || !out) return VT_ERR_ARG;
+    *out = ctx->tuning;
+    return VT_OK;
+}
+
+int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t)
+{
+    if (!ctx || !t) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        check_tuning(ctx, t);
+        ctx->tuning = *t;
+    });
+}
+
+// Coordinate descent over the bucket bounds (one chunk or up to two per row at each group
+// size) and the per-bucket block cap, each candidate timed with HIP events over `reps`
+// sweeps of the same state; a candidate must win by 1% to replace the incumbent.
+int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning* out)
+{
+    if (!ctx || !g || prep < 0 || reps < 1) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        vt_phase_params p;
+        std::memset(&p, 0, sizeof(p));
+        p.lower = -1.0;
+        p.thresh = 1.0E-3;
+        const vt_tuning start = ctx->tuning;
+        vt_phase* st = phase_begin(ctx, g, &p);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            for (int i = 0; i < prep && !st->done; i++) phase_step(ctx, st);
+            VT_HIP(ctx, hipEventCreate(&e0));
+            VT_HIP(ctx, hipEventCreate(&e1));
+            const bool prof = ctx->profiling;
+            ctx->profiling = false;
+            auto time_it = [&](const vt_tuning& t) {
+                ctx->tuning = t;
+                st->ph.build_lists(nullptr, 1, false);
+                st->ph.sweep(st->ph.sweeps[0]);
+                VT_HIP(ctx, hipEventRecord(e0, ctx->stream));
+                for (int r = 0; r < reps; r++) st->ph.sweep(st->ph.sweeps[0]);
+                VT_HIP(ctx, hipEventRecord(e1, ctx->stream));
+                VT_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.f;
+                VT_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                return (double)ms / reps;
+            };
+            vt_tuning best = start;
+            double tb = time_it(best);
+            for (int b = 0; b < 5; b++) {
+                const int G = 4 << b;
+                for (int cand : {G - 1, (3 * G) / 2 - 1, 2 * G - 1}) {
+                    if (cand == best.bucket_max_degree[b]) continue;
+                    vt_tuning t = best;
+                    t.bucket_max_degree[b] = cand;
+                    const double tt = time_it(t);
+                    if (tt < 0.99 * tb) {
+                        best = t;
+                        tb = tt;
+                    }
+                }
+            }
+            for (int cap : {2048, 4096, 8192, 16384, 65536}) {
+                if (cap == best.blocks_per_bucket) continue;
+                vt_tuning t = best;
+                t.blocks_per_bucket = cap;
+                const double tt = time_it(t);
+                if (tt < 0.99 * tb) {
+                    best = t;
+                    tb = tt;
+                }
+            }
+            ctx->profiling = prof;
+            ctx->tuning = best;
+        } catch (...) {
+            ctx->tuning = start;
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            delete st;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        delete st;
+        sync(ctx);
+        if (out) *out = ctx->tuning;
     });
 }
 
