@@ -22,6 +22,7 @@
  *                          another radius and an O(n^2) pair test; its BA needs NetworKit)
  *   vt_run                 the phase loop of main.cpp:242-513
  *   vt_batch_*             the same loop over many graphs at once (no reference counterpart)
+ *   vt_server_*            a request loop running such jobs concurrently (no reference counterpart)
  */
 #ifndef VITE_AMD_H
 #define VITE_AMD_H
@@ -233,6 +234,44 @@ int vt_batch_info(const vt_batch* b, int32_t* n, int64_t* total_nv, int64_t* tot
 /* results: n entries, in the order the graphs were given.  stats may be NULL. */
 int vt_batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* p, vt_run_result* results, vt_batch_stats* stats);
 int vt_batch_free(vt_ctx* ctx, vt_batch* b);
+
+/* ---- serving form (no reference counterpart: Vite runs one job per process) ------------ */
+/* A request loop over several slots (worker threads, each with its own context and HIP
+ * stream, so jobs of different slots overlap on the device).  A request generates a seeded
+ * graph on the device and runs the phase loop on it.  Requests are admitted in arrival
+ * order when a slot is free and their HBM reservation (vt_request_reservation) fits beside
+ * those of the running jobs; a job cannot allocate beyond its reservation.  A result does
+ * not depend on what else runs: every job is computed alone on its own stream. */
+#define VT_REQ_RGG 0  /* seeded RGG (vt_gen_rgg): param = extra edge percent         */
+#define VT_REQ_BA 1   /* seeded Barabasi-Albert (vt_gen_ba): param = attachments m    */
+typedef struct {
+    int32_t kind;
+    uint64_t seed;
+    int64_t n;
+    double param;
+    vt_run_params run;
+} vt_request;
+typedef struct {
+    int32_t status;           /* VT_OK or the failing call's code                          */
+    int32_t slot;             /* slot that ran it                                          */
+    double submit_s;          /* server clock, seconds since vt_server_create              */
+    double start_s;           /* admitted to a slot                                        */
+    double end_s;             /* finished                                                  */
+    uint64_t reserved_bytes;  /* HBM reserved at admission                                 */
+    uint64_t peak_bytes;      /* HBM high-water mark of the job                            */
+    int64_t nv, ne;           /* size of the generated graph                               */
+} vt_request_info;
+typedef struct vt_server vt_server;
+/* Bytes the server reserves for a request (an upper bound on what its job allocates). */
+uint64_t vt_request_reservation(const vt_request* req);
+int vt_server_create(int device, uint64_t hbm_budget_bytes, int32_t slots, vt_server** out);
+/* Queues a request; VT_ERR_BUDGET if its reservation exceeds the whole budget. */
+int vt_server_submit(vt_server* s, const vt_request* req, int64_t* ticket);
+/* Blocks until the request has finished; returns its status.  Each ticket is waited once. */
+int vt_server_wait(vt_server* s, int64_t ticket, vt_run_result* res, vt_request_info* info);
+const char* vt_server_last_error(const vt_server* s);
+/* Finishes every queued request, then stops the slots. */
+int vt_server_destroy(vt_server* s);
 
 #ifdef __cplusplus
 }

@@ -90,3 +90,20 @@ def test_cli_missing_file_is_fatal(tmp_path):
     r = subprocess.run([CLI, "-f", str(tmp_path / "missing.bin")], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "Error opening file" in r.stderr
+
+
+def test_server_without_gpu_fails_cleanly():
+    import torch  # noqa: F401
+    import vite_amd
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(vite_amd.VtError):
+        vite_amd.Server(slots=2)
+
+
+def test_request_reservation_grows_with_size():
+    import vite_amd
+    a = vite_amd.request_reservation("rgg", 1, 1 << 10, 0.0)
+    b = vite_amd.request_reservation("rgg", 1, 1 << 20, 0.0)
+    c = vite_amd.request_reservation("ba", 1, 1 << 20, 8)
+    assert 0 < a < b and 0 < c

@@ -1,0 +1,83 @@
+"""GPU: the serving form (vt_server_*).
+
+Each request's result is the same alone and in the mix (exact: every job runs alone on its
+own stream with the same kernels), equals vt_run on the same generated graph, and the
+admission rule holds: arrival order, and running reservations never above the budget, with
+every job's measured HBM high-water mark inside its reservation.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import vite_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(seed=5, count=24):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(2.0 ** rng.uniform(8, 12.5)))
+        if k % 2 == 0:
+            out.append(("rgg", 1000 + k, n, float(rng.choice([0.0, 1.0]))))
+        else:
+            out.append(("ba", 2000 + k, n, int(rng.integers(1, 9))))
+    return out
+
+
+RUN_KW = [dict(threshold_cycling=True), dict()]
+
+
+def _same(a, b):
+    assert a.call_q == b.call_q
+    assert a.call_iters == b.call_iters
+    assert a.level_nv == b.level_nv
+    assert a.final_q == b.final_q
+
+
+@pytest.mark.parametrize("kw", RUN_KW, ids=["cycling", "plain"])
+def test_mix_equals_alone(ctx, kw):
+    reqs = _requests()
+    with vite_amd.Server(slots=4) as srv:
+        tickets = [srv.submit(*r, **kw) for r in reqs]
+        mixed = [srv.wait(t) for t in tickets]
+    with vite_amd.Server(slots=1) as one:
+        alone = [one.wait(one.submit(*r, **kw)) for r in reqs]
+    for (rm, im), (ra, ia) in zip(mixed, alone):
+        assert im["status"] == 0 and ia["status"] == 0
+        assert (im["nv"], im["ne"]) == (ia["nv"], ia["ne"])
+        _same(rm, ra)
+    # and equal to the one-graph entry point on the same generated graph
+    for (kind, seed, n, p), (rm, _) in list(zip(reqs, mixed))[:6]:
+        g = ctx.gen_rgg(seed, n, p) if kind == "rgg" else ctx.gen_ba(seed, n, p)
+        _same(rm, g.run(**kw))
+        g.free()
+
+
+def test_admission_order_and_budget(ctx):
+    reqs = _requests(seed=9, count=16)
+    res = [vite_amd.request_reservation(*r, threshold_cycling=True) for r in reqs]
+    budget = max(res) + min(res)  # the largest job fits only beside the smallest
+    with vite_amd.Server(hbm_budget=budget, slots=4) as srv:
+        tickets = [srv.submit(*r, threshold_cycling=True) for r in reqs]
+        infos = [srv.wait(t)[1] for t in tickets]
+    starts = [i["start_s"] for i in infos]
+    assert starts == sorted(starts)  # admitted in arrival order
+    for i in infos:
+        assert i["status"] == 0
+        assert i["peak_bytes"] <= i["reserved_bytes"]
+    # at every job start, the jobs running then fit in the budget together
+    for i in infos:
+        t = i["start_s"]
+        running = [j for j in infos if j["start_s"] <= t < j["end_s"]]
+        assert sum(j["reserved_bytes"] for j in running) <= budget
+
+
+def test_oversized_request_rejected(ctx):
+    with vite_amd.Server(hbm_budget=256 << 20, slots=2) as srv:
+        with pytest.raises(vite_amd.VtError):
+            srv.submit("rgg", 1, 1 << 20, 1.0)
+        r, info = srv.wait(srv.submit("rgg", 2, 300, 0.0))
+        assert info["status"] == 0 and r.final_q > 0

@@ -71,6 +71,17 @@ class _Tuning(C.Structure):
     _fields_ = [("bucket_max_degree", C.c_int32 * 6), ("blocks_per_bucket", C.c_int32)]
 
 
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("n", C.c_int64), ("param", C.c_double),
+                ("run", _RunParams)]
+
+
+class _RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("slot", C.c_int32), ("submit_s", C.c_double), ("start_s", C.c_double),
+                ("end_s", C.c_double), ("reserved_bytes", C.c_uint64), ("peak_bytes", C.c_uint64),
+                ("nv", C.c_int64), ("ne", C.c_int64)]
+
+
 class _BatchStats(C.Structure):
     _fields_ = [("phases", C.c_int32), ("rounds", C.c_int32), ("seconds", C.c_double)]
 
@@ -110,6 +121,12 @@ _SIGS = {
     "vt_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_RunParams), C.POINTER(_RunResult),
                                C.POINTER(_BatchStats)]),
     "vt_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vt_request_reservation": (C.c_uint64, [C.POINTER(_Request)]),
+    "vt_server_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "vt_server_submit": (C.c_int, [C.c_void_p, C.POINTER(_Request), C.POINTER(C.c_int64)]),
+    "vt_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(_RunResult), C.POINTER(_RequestInfo)]),
+    "vt_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "vt_server_destroy": (C.c_int, [C.c_void_p]),
     "vt_ctx_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(_Tuning)]),
     "vt_ctx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(_Tuning)]),
     "vt_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(_Tuning)]),
@@ -412,6 +429,63 @@ class Batch:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+REQ_RGG, REQ_BA = 0, 1
+
+
+def _request(kind, seed, n, param, **run_kw):
+    k = {"rgg": REQ_RGG, "ba": REQ_BA}[kind] if isinstance(kind, str) else kind
+    return _Request(k, seed, n, float(param), _run_params(**run_kw))
+
+
+def request_reservation(kind, seed, n, param, **run_kw) -> int:
+    return int(_lib.vt_request_reservation(C.byref(_request(kind, seed, n, param, **run_kw))))
+
+
+class Server:
+    """Serving loop (vt_server_*): requests admitted in arrival order by slot and HBM
+    budget, run concurrently on per-slot streams."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 200 << 30, slots: int = 4):
+        h = C.c_void_p()
+        rc = _lib.vt_server_create(device, hbm_budget, slots, C.byref(h))
+        if rc != VT_OK:
+            raise VtError(rc, "vt_server_create failed")
+        self.h = h
+
+    def _check(self, rc):
+        if rc != VT_OK:
+            msg = _lib.vt_server_last_error(self.h)
+            raise VtError(rc, msg.decode() if msg else "")
+
+    def submit(self, kind, seed, n, param, **run_kw) -> int:
+        t = C.c_int64()
+        self._check(_lib.vt_server_submit(self.h, C.byref(_request(kind, seed, n, param, **run_kw)), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int):
+        """(RunResult, info dict) of a finished request."""
+        res, info = _RunResult(), _RequestInfo()
+        self._check(_lib.vt_server_wait(self.h, ticket, C.byref(res), C.byref(info)))
+        return _run_result(res), {f: getattr(info, f) for f, _ in _RequestInfo._fields_}
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, None
+            _lib.vt_server_destroy(h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

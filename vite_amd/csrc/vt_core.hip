@@ -34,17 +34,20 @@ void* dalloc(vt_ctx* ctx, size_t bytes)
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
         fail(ctx, VT_ERR_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu in use of %llu)",
              bytes, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
+    // Stream-ordered allocation: unlike hipFree, hipFreeAsync does not wait for the whole
+    // device, so contexts on other streams (the serving slots) keep running.
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) fail(ctx, VT_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
+    if (e != hipSuccess) fail(ctx, VT_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
     ctx->in_use += bytes;
+    if (ctx->in_use > ctx->peak) ctx->peak = ctx->in_use;
     return p;
 }
 
 void dfree(vt_ctx* ctx, void* p, size_t bytes)
 {
     if (!p) return;
-    (void)hipFree(p);
+    (void)hipFreeAsync(p, ctx->stream);
     ctx->in_use -= std::min<uint64_t>(ctx->in_use, bytes);
 }
 
@@ -525,6 +528,7 @@ int vt_ctx_destroy(vt_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     release_scratch(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->pending) { ctx->event_pool.push_back(p.e0); ctx->event_pool.push_back(p.e1); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

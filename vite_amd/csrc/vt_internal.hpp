@@ -43,6 +43,7 @@ struct vt_ctx {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t in_use = 0;
+    uint64_t peak = 0;  // high-water mark of in_use since the last reset
     std::string err;
     // reusable scratch buffers keyed by purpose
     std::map<std::string, std::pair<void*, size_t>> scratch;

@@ -1,0 +1,230 @@
+// Serving form: a request loop that runs community-detection jobs (generate a seeded graph
+// on the device, then the whole phase loop of main.cpp:242-513) on several slots at once.
+// Each slot is a worker thread with its own context, hence its own HIP stream, so the jobs
+// of different slots overlap on the device.  Requests are admitted strictly in arrival
+// order, as soon as a slot is free and the request's HBM reservation fits beside the
+// reservations of the running jobs; a job cannot allocate beyond its reservation (its
+// context budget), so the running jobs together never exceed the server's budget.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "vt_internal.hpp"
+
+namespace {
+
+struct Job {
+    int64_t ticket = 0;
+    vt_request req{};
+    uint64_t reserve = 0;
+    bool done = false;
+    vt_run_result res{};
+    vt_request_info info{};
+    std::string err;
+};
+
+}  // namespace
+
+struct vt_server {
+    int device = 0;
+    uint64_t budget = 0;
+    std::vector<vt_ctx*> ctxs;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job*> queue;
+    std::map<int64_t, std::unique_ptr<Job>> jobs;
+    int64_t next_ticket = 1;
+    uint64_t reserved = 0;
+    bool stop = false;
+    std::string err;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace vt {
+namespace {
+
+// Expected directed entries of a request's graph: RGG with radius sqrt(2 ln n / (pi n)) has
+// mean degree about 2 ln n (less near the border), plus the extra-edge share; BA has 2 n m.
+double expected_entries(const vt_request& r)
+{
+    const double n = (double)r.n;
+    if (r.kind == VT_REQ_BA) return 2.0 * n * r.param;
+    return n > 1 ? 2.0 * n * std::log(n) * (1.0 + r.param / 100.0) : 0.0;
+}
+
+void run_job(vt_server* s, int slot, Job* j)
+{
+    vt_ctx* ctx = s->ctxs[slot];
+    ctx->budget = j->reserve;
+    ctx->peak = ctx->in_use;
+    vt_graph* g = nullptr;
+    int rc = j->req.kind == VT_REQ_BA ? vt_gen_ba(ctx, j->req.seed, j->req.n, (int64_t)j->req.param, &g)
+                                      : vt_gen_rgg(ctx, j->req.seed, j->req.n, j->req.param, &g);
+    if (rc == VT_OK) {
+        vt_graph_info(g, &j->info.nv, &j->info.ne);
+        rc = vt_run(ctx, g, &j->req.run, &j->res);
+    }
+    if (rc != VT_OK) j->err = vt_last_error(ctx);
+    if (g) vt_graph_free(ctx, g);
+    // nothing of this job stays on the slot: its scratch goes back before the next admission
+    release_scratch(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
+    j->info.status = rc;
+    j->info.peak_bytes = ctx->peak;
+    ctx->budget = 0;
+}
+
+void worker(vt_server* s, int slot)
+{
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        Job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                return (s->stop && s->queue.empty()) ||
+                       (!s->queue.empty() && s->reserved + s->queue.front()->reserve <= s->budget);
+            });
+            if (s->queue.empty()) return;  // stopping and drained
+            j = s->queue.front();
+            s->queue.pop_front();
+            s->reserved += j->reserve;
+            j->info.start_s = s->now();
+            j->info.slot = slot;
+        }
+        s->cv_work.notify_all();  // the next request may fit too
+        run_job(s, slot, j);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->reserved -= j->reserve;
+            j->info.end_s = s->now();
+            j->done = true;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+}  // namespace vt
+
+using namespace vt;
+
+extern "C" {
+
+uint64_t vt_request_reservation(const vt_request* r)
+{
+    if (!r || r->n < 1) return 0;
+    // CSR (16 B per entry + 8 per vertex) held by the caller graph and a level graph, the
+    // rebuild's key/weight double buffers (32 B per entry) and sort scratch, generator
+    // buffers, the phase state (about 200 B per vertex), with a 25% margin on the entry count.
+    const double e = 1.25 * expected_entries(*r) + 1024.0;
+    return (uint64_t)(112.0 * e + 320.0 * (double)r->n) + (64ull << 20);
+}
+
+int vt_server_create(int device, uint64_t hbm_budget_bytes, int32_t slots, vt_server** out)
+{
+    if (!out || slots < 1 || slots > 16 || hbm_budget_bytes == 0) return VT_ERR_ARG;
+    *out = nullptr;
+    vt_server* s = new vt_server();
+    s->device = device;
+    s->budget = hbm_budget_bytes;
+    for (int i = 0; i < slots; i++) {
+        vt_ctx* c = nullptr;
+        const int rc = vt_ctx_create(device, 0, &c);
+        if (rc != VT_OK) {
+            for (vt_ctx* x : s->ctxs) vt_ctx_destroy(x);
+            delete s;
+            return rc;
+        }
+        s->ctxs.push_back(c);
+    }
+    for (int i = 0; i < slots; i++) s->workers.emplace_back(worker, s, i);
+    *out = s;
+    return VT_OK;
+}
+
+int vt_server_submit(vt_server* s, const vt_request* req, int64_t* ticket)
+{
+    if (!s || !req || !ticket) return VT_ERR_ARG;
+    if (req->n < 1 || (req->kind != VT_REQ_RGG && req->kind != VT_REQ_BA) ||
+        (req->kind == VT_REQ_BA && req->param < 1.0)) {
+        s->err = "vt_server_submit: bad request";
+        return VT_ERR_ARG;
+    }
+    auto j = std::make_unique<Job>();
+    j->req = *req;
+    j->reserve = vt_request_reservation(req);
+    if (j->reserve > s->budget) {
+        s->err = "vt_server_submit: request needs " + std::to_string(j->reserve) + " bytes, more than the budget";
+        return VT_ERR_BUDGET;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return VT_ERR_STATE;
+        j->ticket = s->next_ticket++;
+        j->info.submit_s = s->now();
+        j->info.reserved_bytes = j->reserve;
+        *ticket = j->ticket;
+        s->queue.push_back(j.get());
+        s->jobs[j->ticket] = std::move(j);
+    }
+    s->cv_work.notify_all();
+    return VT_OK;
+}
+
+int vt_server_wait(vt_server* s, int64_t ticket, vt_run_result* res, vt_request_info* info)
+{
+    if (!s) return VT_ERR_ARG;
+    std::unique_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(ticket);
+        if (it == s->jobs.end()) {
+            s->err = "vt_server_wait: unknown ticket";
+            return VT_ERR_ARG;
+        }
+        Job* p = it->second.get();
+        s->cv_done.wait(lk, [&] { return p->done; });
+        j = std::move(it->second);
+        s->jobs.erase(it);
+    }
+    if (res) *res = j->res;
+    if (info) *info = j->info;
+    if (j->info.status != VT_OK) s->err = j->err;
+    return j->info.status;
+}
+
+const char* vt_server_last_error(const vt_server* s)
+{
+    return s ? s->err.c_str() : "null server";
+}
+
+int vt_server_destroy(vt_server* s)
+{
+    if (!s) return VT_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    for (vt_ctx* c : s->ctxs) vt_ctx_destroy(c);
+    delete s;
+    return VT_OK;
+}
+
+}  // extern "C"
