@@ -21,6 +21,7 @@ state (oracle/_ref, OpenMP threads), with the GPU targets checked against it.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -146,6 +147,60 @@ def batched_form(ctx, args):
             "workload": "RGG n log-uniform in [2^8, 2^16], 1% extra edges on half, threshold cycling"}
 
 
+def serving_form(args):
+    """The serving form: a Poisson stream of requests (seeded RGG or BA, 50/50, n log-uniform
+    in [2^10, 2^20], threshold cycling) against a server with args.serve_slots slots and a
+    200 GB budget, and the same stream against the sequential solver (one slot).  The
+    offered load is args.serve_load times the sequential solver's capacity, measured first
+    on the same requests back to back."""
+    import numpy as np
+    import vite_amd
+
+    rng = np.random.default_rng(args.seed + 5)
+    count = args.serve_requests
+    reqs = []
+    for k in range(count):
+        n = int(round(2.0 ** rng.uniform(10.0, 20.0)))
+        reqs.append(("rgg", 7000 + k, n, 0.0) if rng.random() < 0.5 else ("ba", 9000 + k, n, 8))
+    kw = dict(threshold_cycling=True)
+    budget = 200 * 10**9
+
+    def replay(slots, arrivals):
+        with vite_amd.Server(hbm_budget=budget, slots=slots) as srv:
+            t0 = time.perf_counter()
+            tickets = []
+            for r, at in zip(reqs, arrivals):
+                while True:
+                    dt = at - (time.perf_counter() - t0)
+                    if dt <= 0:
+                        break
+                    time.sleep(min(dt, 0.01))
+                tickets.append(srv.submit(*r, **kw))
+            out = [srv.wait(t) for t in tickets]
+        lat = np.array([i["end_s"] - i["submit_s"] for _, i in out])
+        span = max(i["end_s"] for _, i in out) - min(i["submit_s"] for _, i in out)
+        return out, {"p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                     "graphs_per_s": count / span, "makespan_s": span}
+
+    # capacity of the sequential solver: the requests back to back
+    _, closed = replay(1, [0.0] * count)
+    lam = args.serve_load * closed["graphs_per_s"]
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=count))
+    seq_out, seq = replay(1, arrivals)
+    mix_out, mix = replay(args.serve_slots, arrivals)
+    same = all(a.call_q == b.call_q and a.call_iters == b.call_iters and a.final_q == b.final_q
+               for (a, _), (b, _) in zip(seq_out, mix_out))
+    ok = all(i["status"] == 0 for _, i in mix_out)
+    log(f"serving: offered {lam:.1f} req/s; sequential p50 {seq['p50_ms']:.1f} ms p99 {seq['p99_ms']:.1f} ms "
+        f"{seq['graphs_per_s']:.1f} graphs/s; {args.serve_slots} slots p50 {mix['p50_ms']:.1f} ms "
+        f"p99 {mix['p99_ms']:.1f} ms {mix['graphs_per_s']:.1f} graphs/s; identical={same}")
+    return {"requests": count, "offered_per_s": lam, "slots": args.serve_slots, "budget_bytes": budget,
+            "p50_ms": mix["p50_ms"], "p99_ms": mix["p99_ms"], "graphs_per_s": mix["graphs_per_s"],
+            "sequential": seq, "sequential_capacity_per_s": closed["graphs_per_s"],
+            "identical_alone_vs_mix": same, "all_ok": ok,
+            "workload": "Poisson arrivals; RGG or BA (m=8) 50/50, n log-uniform in [2^10, 2^20]; threshold cycling"}
+
+
 def measured_traffic(kernel, workload_tag):
     """HBM bytes per launch of `kernel` from the committed rocprof counter summary of this
     workload (FETCH_SIZE x FETCH_SCALE + WRITE_SIZE x WRITE_SCALE), or None."""
@@ -155,7 +210,8 @@ def measured_traffic(kernel, workload_tag):
     vals = {}
     for line in open(path):
         parts = [p.strip() for p in line.split("|")]
-        if len(parts) == 4 and parts[0].endswith("k_" + kernel) and parts[1] in ("FETCH_SIZE", "WRITE_SIZE"):
+        name = re.sub(r"<.*>", "", parts[0]) if parts else ""
+        if len(parts) == 4 and name.endswith("k_" + kernel) and parts[1] in ("FETCH_SIZE", "WRITE_SIZE"):
             vals[parts[1]] = float(parts[3])
     if len(vals) != 2:
         return None, None
@@ -181,6 +237,10 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batched-form run (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=256,
                     help="graphs of the batch also run one graph per launch (the comparison)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests in the serving run (0 = skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-load", type=float, default=0.9,
+                    help="offered load as a fraction of the sequential solver's measured capacity")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration)")
     args = ap.parse_args()
@@ -271,6 +331,8 @@ def main():
 
     if args.batch_graphs > 0:
         out["batched"] = batched_form(ctx, args)
+    if args.serve_requests > 0:
+        out["serving"] = serving_form(args)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -303,6 +303,45 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 
 // Insert-or-find without an occupancy list: CAS first (one LDS round trip whether the key
 // is new or already present).
+#ifndef VT_DPP_ARGMAX
+#define VT_DPP_ARGMAX 0
+#endif
+constexpr bool kDppArgmax = VT_DPP_ARGMAX;
+
+// (gain, community) argmax step: take the other candidate if its gain is larger, or equal
+// with a smaller id (the reference's order-independent rule, louvain.cpp:2246-2273).
+__device__ __forceinline__ void argmax_take(double& bg, int32_t& bc, double og, int32_t oc)
+{
+    if (og > bg || (og == bg && oc < bc)) {
+        bg = og;
+        bc = oc;
+    }
+}
+
+// One DPP step: every lane combines with the lane the control selects; lanes whose source
+// lies outside the row (or the masked rows) see the identity (0.0, INT_MAX).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void argmax_dpp_step(double& bg, int32_t& bc)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(bg), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(bg), CTRL, ROWS, 0xF, false);
+    const int oc = __builtin_amdgcn_update_dpp(INT_MAX, bc, CTRL, ROWS, 0xF, false);
+    argmax_take(bg, bc, __hiloint2double(hi, lo), oc);
+}
+
+// Argmax over the G lanes of each group, reduced into the group's highest lane with DPP
+// (row_shr within 16-lane rows, then row_bcast15 / row_bcast31 across rows): no LDS traffic.
+template <int G>
+__device__ __forceinline__ void group_argmax_dpp(double& bg, int32_t& bc)
+{
+    if constexpr (G >= 2) argmax_dpp_step<0x111, 0xF>(bg, bc);   // row_shr:1
+    if constexpr (G >= 4) argmax_dpp_step<0x112, 0xF>(bg, bc);   // row_shr:2
+    if constexpr (G >= 8) argmax_dpp_step<0x114, 0xF>(bg, bc);   // row_shr:4
+    if constexpr (G >= 16) argmax_dpp_step<0x118, 0xF>(bg, bc);  // row_shr:8
+    if constexpr (G >= 32) argmax_dpp_step<0x142, 0xA>(bg, bc);  // row_bcast15 into rows 1, 3
+    if constexpr (G >= 64) argmax_dpp_step<0x143, 0xC>(bg, bc);  // row_bcast31 into rows 2, 3
+}
+
 // Table of one group (G lanes, one vertex at a time) in LDS.
 template <int G>
 struct GroupTable {
@@ -572,15 +611,20 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                     }
                 }
             }
-            for (int o = G / 2; o > 0; o >>= 1) {
-                const double og = __shfl_xor(bg, o, G);
-                const int32_t oc = __shfl_xor(bc, o, G);
-                if (og > bg || (og == bg && oc < bc)) {
-                    bg = og;
-                    bc = oc;
+            constexpr int kWriter = kDppArgmax ? G - 1 : 0;
+            if constexpr (kDppArgmax) {
+                group_argmax_dpp<G>(bg, bc);
+            } else {
+                for (int o = G / 2; o > 0; o >>= 1) {
+                    const double og = __shfl_xor(bg, o, G);
+                    const int32_t oc = __shfl_xor(bc, o, G);
+                    if (og > bg || (og == bg && oc < bc)) {
+                        bg = og;
+                        bc = oc;
+                    }
                 }
             }
-            if (lane == 0) {
+            if (lane == kWriter) {
                 int64_t tgt = cc;
                 if (bg > 0.0) {
                     tgt = bc;
