@@ -147,6 +147,48 @@ def batched_form(ctx, args):
             "workload": "RGG n log-uniform in [2^8, 2^16], 1% extra edges on half, threshold cycling"}
 
 
+def skewed_form(ctx, args):
+    """Skewed degrees: seeded Barabasi-Albert graph, n = args.skewed_n, m = 8, unit weights.
+    Local-move sweeps (plain phase 0 after --prep iterations) with the per-kernel split over
+    the degree classes, then the whole loop with coloring (8 colors) and early termination
+    with probability (-t 2 -a 0.25)."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    dg = ctx.gen_ba(args.seed + 3, args.skewed_n, 8)
+    gen_s = time.perf_counter() - t0
+    deg = np.diff(dg.offsets())
+    log(f"skewed: BA n={dg.nv} m=8, {dg.ne} entries, max degree {int(deg.max())} (generated in {gen_s:.2f}s)")
+    st = dg.phase_begin(lower=-1.0, thresh=1e-3)
+    for _ in range(args.prep):
+        st.step()
+    st.sweep()
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st.sweep()
+    wall = (time.perf_counter() - t0) / reps
+    kernels = {}
+    for k in ("local_move", "local_move_mid", "local_move_wide", "local_move_hub"):
+        avg, cnt, _ = ctx.kernel_stats(k)
+        if cnt:
+            kernels[k] = avg
+    ctx.set_profiling(False)
+    st.end(want_cvect=False)
+    t0 = time.perf_counter()
+    r = dg.run(coloring=True, max_colors=8, et_type=2, et_delta=0.25)
+    run_s = time.perf_counter() - t0
+    dg.free()
+    log(f"skewed: sweep {1e3 * wall:.3f} ms ({dg.ne / wall / 1e9:.2f} G edges/s) {kernels}; "
+        f"-c 8 -t 2 -a 0.25 run {run_s:.2f}s Q={r.final_q:.9f}")
+    return {"n": int(len(deg)), "directed_entries": int(deg.sum()), "max_degree": int(deg.max()),
+            "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall, "kernel_avg_ms": kernels,
+            "run_seconds": run_s, "final_q": r.final_q, "levels": len(r.level_q), "colors": r.num_colors,
+            "flags": "-c 8 -t 2 -a 0.25"}
+
+
 def serving_form(args):
     """The serving form: a Poisson stream of requests (seeded RGG or BA, 50/50, n log-uniform
     in [2^10, 2^20], threshold cycling) against a server with args.serve_slots slots and a
@@ -237,6 +279,7 @@ def main():
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batched-form run (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=256,
                     help="graphs of the batch also run one graph per launch (the comparison)")
+    ap.add_argument("--skewed-n", type=int, default=1 << 22, help="BA vertices of the skewed-degree run (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.9,
@@ -329,6 +372,8 @@ def main():
                                   "clustering_seconds": r.clustering_seconds, "flags": "-i"}
         log(f"time to final Q: {ttq:.2f}s q={r.final_q:.9f} levels={len(r.level_q)} iters={r.total_iters}")
 
+    if args.skewed_n > 0:
+        out["skewed"] = skewed_form(ctx, args)
     if args.batch_graphs > 0:
         out["batched"] = batched_form(ctx, args)
     if args.serve_requests > 0:

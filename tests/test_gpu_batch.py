@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 def _mixed_graphs():
     gs = [graphs.graph(n) for n in graphs.ALL]
     gs += [po.gen_rgg(100 + k, n, extra) for k, (n, extra) in
-           enumerate([(2, 0.0), (17, 0.0), (300, 1.0), (1200, 0.0), (4000, 1.0), (700, 10.0)])]
+           enumerate([(9, 5.0), (17, 0.0), (300, 1.0), (1200, 0.0), (4000, 1.0), (700, 10.0)])]
     gs += [po.gen_ba(200 + k, n, m) for k, (n, m) in enumerate([(50, 1), (900, 3), (2500, 8)])]
     return gs
 

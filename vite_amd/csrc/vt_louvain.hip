@@ -304,7 +304,7 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 // Insert-or-find without an occupancy list: CAS first (one LDS round trip whether the key
 // is new or already present).
 #ifndef VT_DPP_ARGMAX
-#define VT_DPP_ARGMAX 0
+#define VT_DPP_ARGMAX 1
 #endif
 constexpr bool kDppArgmax = VT_DPP_ARGMAX;
 
@@ -663,7 +663,7 @@ struct LdsLayout {
 };
 
 #ifndef VT_XCD_RANGES
-#define VT_XCD_RANGES 1
+#define VT_XCD_RANGES 0
 #endif
 constexpr bool kXcdRanges = VT_XCD_RANGES;
 constexpr int kXcds = 8;
