@@ -133,6 +133,11 @@ struct MoveArgs {
     // iterating (null for a single graph).
     const int32_t* gid;
     const unsigned char* gact;
+    // Structure-of-arrays copy of the edges (32-bit tails, weights) and a 32-bit copy of
+    // currComm, read by the group buckets; null when the build reads the records directly.
+    const int32_t* tail32;
+    const double* wgt;
+    const int32_t* curr32;
 };
 
 // Inserts or finds community c; returns its slot.
@@ -303,6 +308,11 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 
 // Insert-or-find without an occupancy list: CAS first (one LDS round trip whether the key
 // is new or already present).
+#ifndef VT_SPLIT_EDGES
+#define VT_SPLIT_EDGES 1
+#endif
+constexpr bool kSplitEdges = VT_SPLIT_EDGES;
+
 #ifndef VT_DPP_ARGMAX
 #define VT_DPP_ARGMAX 1
 #endif
@@ -312,10 +322,10 @@ constexpr bool kDppArgmax = VT_DPP_ARGMAX;
 // with a smaller id (the reference's order-independent rule, louvain.cpp:2246-2273).
 __device__ __forceinline__ void argmax_take(double& bg, int32_t& bc, double og, int32_t oc)
 {
-    if (og > bg || (og == bg && oc < bc)) {
-        bg = og;
-        bc = oc;
-    }
+    // bitwise, not short-circuit: no divergent branches
+    const bool take = (og > bg) | ((og == bg) & (oc < bc));
+    bg = take ? og : bg;
+    bc = take ? oc : bc;
 }
 
 // One DPP step: every lane combines with the lane the control selects; lanes whose source
@@ -343,6 +353,8 @@ __device__ __forceinline__ void group_argmax_dpp(double& bg, int32_t& bc)
 }
 
 // Table of one group (G lanes, one vertex at a time) in LDS.
+// Lanes with nothing to store or read point at the block's trash cells instead, so LDS
+// accesses need no divergent branches (the trash is written and read racily and never used).
 template <int G>
 struct GroupTable {
     using mask_t = typename MaskT<G>::type;
@@ -352,6 +364,10 @@ struct GroupTable {
     uint8_t* occ;     // 2G: occupied slots in insertion order
     double* wst;      // G staged chunk weights; wst[-1] holds -0.0
     uint32_t capmask;
+    int32_t* tkey;    // trash cells shared by the block
+    double* tval;
+    mask_t* tmask;
+    uint8_t* tocc;
 };
 
 // Folds the staged weights of the lanes in `rest` (ascending lane = edge order) onto acc,
@@ -379,7 +395,7 @@ __device__ __forceinline__ double fold_rest(double acc, mask_t rest, const doubl
 }
 
 #ifndef VT_FOLD_UNROLL
-#define VT_FOLD_UNROLL 4
+#define VT_FOLD_UNROLL 2
 #endif
 
 // One chunk of at most G edges of a row already in registers: insert / find each
@@ -410,12 +426,13 @@ __device__ __forceinline__ void fold_chunk(const GroupTable<G>& t, double* ecc_c
         slot = (int32_t)h;
     }
     const mask_t fm = (mask_t)((__ballot(fresh) >> gbase) & gmask);
-    if (fresh) t.occ[nocc + __popcll((unsigned long long)(fm & below))] = (uint8_t)slot;
+    *(fresh ? &t.occ[nocc + __popcll((unsigned long long)(fm & below))] : t.tocc) = (uint8_t)slot;
     nocc += __popcll((unsigned long long)fm);
     t.wst[lane] = w;
-    if (valid) atomicOr(&t.masks[slot], (mask_t)1 << lane);
+    mask_t* mp = valid ? &t.masks[slot] : t.tmask;
+    atomicOr(mp, (mask_t)1 << lane);
     wave_sync();
-    const mask_t m = valid ? t.masks[slot] : (mask_t)0;
+    const mask_t m = *mp;
     wave_sync();
     if (valid && (m & below) == 0) {
         double acc = t.vals[slot];
@@ -468,7 +485,8 @@ __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t
         const bool live = !(a.active && !a.active[v]);
         h.e0 = a.offs[v];
         h.deg = live ? (int32_t)(a.offs[v + 1] - h.e0) : -1;
-        h.cc = (int32_t)a.curr[v];
+        if constexpr (kSplitEdges) h.cc = a.curr32[v];
+        else h.cc = (int32_t)a.curr[v];
         h.kv = a.vdeg[v];
         if constexpr (BATCH) {
             const int32_t gg = a.gid[v];
@@ -502,29 +520,45 @@ template <int G, bool BATCH>
 __device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader<BATCH>& h, int lane, RowRegs& r)
 {
     const int64_t j0 = h.e0 + lane, j1 = h.e0 + G + lane;
-    const int32_t* words = reinterpret_cast<const int32_t*>(a.edges);
-    const double* wts = reinterpret_cast<const double*>(a.edges);
     r.ok0 = lane < h.deg;
     r.ok1 = G + lane < h.deg;
     r.t0 = r.t1 = -1;
     r.w0 = r.w1 = 0.0;
-    if (r.ok0) {
-        r.t0 = words[4 * j0];
-        r.w0 = wts[2 * j0 + 1];
-    }
-    if (r.ok1) {
-        r.t1 = words[4 * j1];
-        r.w1 = wts[2 * j1 + 1];
+    if constexpr (kSplitEdges) {
+        if (r.ok0) {
+            r.t0 = a.tail32[j0];
+            r.w0 = a.wgt[j0];
+        }
+        if (r.ok1) {
+            r.t1 = a.tail32[j1];
+            r.w1 = a.wgt[j1];
+        }
+    } else {
+        const int32_t* words = reinterpret_cast<const int32_t*>(a.edges);
+        const double* wts = reinterpret_cast<const double*>(a.edges);
+        if (r.ok0) {
+            r.t0 = words[4 * j0];
+            r.w0 = wts[2 * j0 + 1];
+        }
+        if (r.ok1) {
+            r.t1 = words[4 * j1];
+            r.w1 = wts[2 * j1 + 1];
+        }
     }
 }
 
 __device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, RowRegs& r)
 {
-    const int32_t* cw = reinterpret_cast<const int32_t*>(a.curr);
     r.c0 = 0;
     r.c1 = 0;
-    if (r.ok0) r.c0 = cw[2 * (int64_t)r.t0];
-    if (r.ok1) r.c1 = cw[2 * (int64_t)r.t1];
+    if constexpr (kSplitEdges) {
+        if (r.ok0) r.c0 = a.curr32[r.t0];
+        if (r.ok1) r.c1 = a.curr32[r.t1];
+    } else {
+        const int32_t* cw = reinterpret_cast<const int32_t*>(a.curr);
+        if (r.ok0) r.c0 = cw[2 * (int64_t)r.t0];
+        if (r.ok1) r.c1 = cw[2 * (int64_t)r.t1];
+    }
     r.self0 = r.ok0 && r.t0 == v;
     r.self1 = r.ok1 && r.t1 == v;
 }
@@ -594,22 +628,19 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             int32_t bc = INT_MAX;
             for (int base = 0; base < nocc; base += G) {
                 const bool ok = base + lane < nocc;
-                const int sl = ok ? t.occ[base + lane] : 0;
-                const int32_t ck = ok ? t.keys[sl] : -1;
-                const double ey = t.vals[sl];
-                if (ok) {
-                    t.keys[sl] = -1;
-                    t.vals[sl] = sent_value();
-                }
-                const bool cand = ok && (int64_t)ck != cc;
-                const double ay = cand ? a.cdeg[ck] : 0.0;
-                if (cand) {
-                    const double gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
-                    if (gain > 0.0 && (gain > bg || (gain == bg && ck < bc))) {
-                        bg = gain;
-                        bc = ck;
-                    }
-                }
+                const int sl = t.occ[ok ? base + lane : 0];
+                int32_t* kp = ok ? &t.keys[sl] : t.tkey;
+                double* vp = ok ? &t.vals[sl] : t.tval;
+                const int32_t ck = *kp;
+                const double ey = *vp;
+                *kp = -1;
+                *vp = sent_value();
+                const bool cand = ok & ((int64_t)ck != cc);
+                const double ay = a.cdeg[cand ? ck : (int32_t)cc];
+                const double gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
+                const bool take = cand & (gain > 0.0) & ((gain > bg) | ((gain == bg) & (ck < bc)));
+                bg = take ? gain : bg;
+                bc = take ? ck : bc;
             }
             constexpr int kWriter = kDppArgmax ? G - 1 : 0;
             if constexpr (kDppArgmax) {
@@ -678,7 +709,8 @@ struct GroupLds {
     static constexpr size_t masks = align16(vals + sizeof(double) * NG * CAP);
     static constexpr size_t wst = align16(masks + sizeof(mask_t) * NG * CAP);
     static constexpr size_t ecc = align16(wst + sizeof(double) * NG * (G + 1));
-    static constexpr size_t occ = align16(ecc + sizeof(double) * NG);
+    static constexpr size_t trash = align16(ecc + sizeof(double) * NG);  // val, mask, key, occ
+    static constexpr size_t occ = trash + 32;
     static constexpr size_t bytes = align16(occ + (size_t)NG * 2 * G);
 };
 
@@ -708,8 +740,10 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
         }
         if (threadIdx.x < NG) wst[threadIdx.x * (G + 1)] = -0.0;
         __syncthreads();
+        char* tr = lds + L::trash;
         GroupTable<G> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, (uint8_t*)(lds + L::occ) + g * 2 * G,
-                        wst + g * (G + 1) + 1, (uint32_t)(CAP - 1)};
+                        wst + g * (G + 1) + 1, (uint32_t)(CAP - 1), (int32_t*)(tr + 16), (double*)tr,
+                        (mask_t*)(tr + 8), (uint8_t*)(tr + 20)};
         double* ecc_cell = (double*)(lds + L::ecc) + g;
         if (kXcdRanges && nblk % kXcds == 0) {
             // Workgroups are dealt to the XCDs round-robin, so block blk runs on XCD
@@ -1168,6 +1202,22 @@ __global__ void k_rotate_et2(int64_t nv, int num_iters, double one_minus_delta, 
     }
 }
 
+__global__ void k_split_edges(const DEdge* __restrict__ edges, int64_t ne, int32_t* __restrict__ tail32,
+                              double* __restrict__ wgt)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ne; j += (int64_t)gridDim.x * blockDim.x) {
+        const DEdge e = edges[j];
+        tail32[j] = (int32_t)e.tail;
+        wgt[j] = e.weight;
+    }
+}
+
+__global__ void k_narrow_ids(const int64_t* __restrict__ src, int64_t n, int32_t* __restrict__ dst)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (int32_t)src[i];
+}
+
 // ---- batch of disjoint graphs ------------------------------------------------------------
 
 __global__ void k_seg_constant(const double* __restrict__ sums, int32_t ng, double* __restrict__ constant)
@@ -1275,8 +1325,27 @@ struct Phase {
     const int32_t* gid = nullptr;
     const unsigned char* gact = nullptr;
     double* gconst = nullptr;
+    // Narrow copies read by the group buckets: edges split once per phase, currComm
+    // narrowed before every sweep (it changes between sweeps).
+    DBuf<int32_t> tail32, curr32;
+    DBuf<double> wgt;
 
     Phase(vt_ctx* c, const vt_graph* gr) : ctx(c), g(gr), nv(gr->nv) {}
+
+    void prepare_narrow()
+    {
+        if constexpr (!kSplitEdges) return;
+        if (!tail32.p) {
+            const size_t ne1 = (size_t)std::max<int64_t>(g->ne, 1);
+            tail32 = DBuf<int32_t>(ctx, ne1);
+            wgt = DBuf<double>(ctx, ne1);
+            curr32 = DBuf<int32_t>(ctx, (size_t)std::max<int64_t>(nv, 1));
+            if (g->ne)
+                VT_LAUNCH(ctx, "split_edges", k_split_edges, grid_for(g->ne, 256, 16384), 256, 0,
+                          (const DEdge*)g->edges, g->ne, tail32.p, wgt.p);
+        }
+        if (nv) VT_LAUNCH(ctx, "narrow_curr", k_narrow_ids, grid_for(nv, 256, 8192), 256, 0, (const int64_t*)curr.p, nv, curr32.p);
+    }
 
     BucketBounds bounds() const
     {
@@ -1437,11 +1506,15 @@ struct Phase {
         a.inactive_count = ucount.p;
         a.gid = gid;
         a.gact = gact;
+        a.tail32 = tail32.p;
+        a.wgt = wgt.p;
+        a.curr32 = curr32.p;
         return a;
     }
 
     void sweep(const ColorSweep& cs)
     {
+        prepare_narrow();
         const MoveArgs a = move_args();
         if (cs.lpv_n) {
             const int64_t nblk = std::min<int64_t>((cs.lpv_n + 63) / 64, 16384);
