@@ -518,6 +518,14 @@ int vt_ctx_create(int device, uint64_t hbm_budget_bytes, vt_ctx** out)
         delete c;
         return VT_ERR_HIP;
     }
+    // Keep freed blocks in the device's stream-ordered pool instead of unmapping them at
+    // every synchronisation: a phase loop frees and reallocates the same large buffers
+    // (phase state, rebuild keys) level after level.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+        uint64_t keep = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     *out = c;
     return VT_OK;
 }

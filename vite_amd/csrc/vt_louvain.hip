@@ -1024,6 +1024,161 @@ __global__ void __launch_bounds__(64) k_move_hbm(MoveArgs a, const int32_t* __re
     if (lane == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
 
+// ---- hub rows by sorting ----------------------------------------------------------------
+// Rows above the wide bound: every (hub, neighbour community) pair becomes a key, one
+// stable radix sort groups each hub's edges by community with edge order kept inside a
+// group, and a workgroup per hub folds every group (a run) left to right, scores it and
+// reduces the argmax.  The same left folds in CSR order as distBuildLocalMapCounter
+// (louvain.cpp:2419-2466), so the same gains as the table kernels.
+
+// Keys (hub j << 32 | community of the tail) and values (edge index) for hub j's row.
+__global__ void k_hub_keys(const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
+                           const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
+                           const int64_t* __restrict__ curr, unsigned long long* __restrict__ keys,
+                           uint32_t* __restrict__ vals)
+{
+    for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
+        const int64_t q0 = hoffs[j], n = hoffs[j + 1] - q0, e0 = offs[hubs[j]];
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int64_t k = e0 + i;
+            keys[q0 + i] = ((unsigned long long)j << 32) | (uint32_t)curr[edges[k].tail];
+            vals[q0 + i] = (uint32_t)k;
+        }
+    }
+}
+
+__global__ void k_hub_heads(const unsigned long long* __restrict__ keys, int64_t n, int32_t* __restrict__ head)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        head[q] = (q == 0 || keys[q] != keys[q - 1]) ? 1 : 0;
+}
+
+// starts[r] = first position of run r, and starts[nruns] = n.
+__global__ void k_hub_starts(const int32_t* __restrict__ head, const int32_t* __restrict__ pos, int64_t n,
+                             int32_t* __restrict__ starts)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        if (head[q]) starts[pos[q]] = (int32_t)q;
+        if (q == n - 1) starts[pos[q] + head[q]] = (int32_t)n;
+    }
+}
+
+__global__ void k_hub_degrees(const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ offs,
+                              int64_t* __restrict__ deg)
+{
+    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j <= nh; j += gridDim.x * blockDim.x)
+        deg[j] = j < nh ? offs[hubs[j] + 1] - offs[hubs[j]] : 0;
+}
+
+// Left fold of a run's weights: the first as is, or from 0.0 for the vertex's own community
+// (whose counter the reference seeds with 0.0); the self-loop weights folded from 0.0 too.
+__device__ __forceinline__ double hub_run_fold(const DEdge* __restrict__ edges, const uint32_t* __restrict__ vals,
+                                               int64_t b, int64_t e, bool own, int32_t v, double& self)
+{
+    double acc = own ? 0.0 : edges[vals[b]].weight;
+    int64_t k = own ? b : b + 1;
+    for (; k + 4 <= e; k += 4) {
+        const DEdge x0 = edges[vals[k]], x1 = edges[vals[k + 1]], x2 = edges[vals[k + 2]], x3 = edges[vals[k + 3]];
+        acc = acc + x0.weight;
+        if (own && x0.tail == v) self = self + x0.weight;
+        acc = acc + x1.weight;
+        if (own && x1.tail == v) self = self + x1.weight;
+        acc = acc + x2.weight;
+        if (own && x2.tail == v) self = self + x2.weight;
+        acc = acc + x3.weight;
+        if (own && x3.tail == v) self = self + x3.weight;
+    }
+    for (; k < e; k++) {
+        const DEdge x = edges[vals[k]];
+        acc = acc + x.weight;
+        if (own && x.tail == v) self = self + x.weight;
+    }
+    return acc;
+}
+
+constexpr int kHubBlock = 256;
+
+__global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_t* __restrict__ hubs, int32_t nh,
+                                                        const int64_t* __restrict__ hoffs,
+                                                        const unsigned long long* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const int32_t* __restrict__ pos,
+                                                        const int32_t* __restrict__ starts)
+{
+    __shared__ double s_ecc, s_self;
+    __shared__ double s_bg[kHubBlock / 64];
+    __shared__ int32_t s_bc[kHubBlock / 64];
+    unsigned inactive = 0;
+    const double constant = *a.constant;
+    const int64_t total = hoffs[nh];
+    const int32_t nruns = total ? pos[total - 1] + (int32_t)(keys[total - 1] != (total > 1 ? keys[total - 2] : ~0ull)) : 0;
+    for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
+        const int32_t v = hubs[j];
+        if (a.active && !a.active[v]) {
+            if (threadIdx.x == 0) {
+                a.cw[v] = a.frozen[v];  // louvain.cpp:101-104
+                inactive++;
+            }
+            continue;
+        }
+        const int32_t r0 = hoffs[j] < total ? pos[hoffs[j]] : nruns;
+        const int32_t r1 = hoffs[j + 1] < total ? pos[hoffs[j + 1]] : nruns;
+        const int64_t cc = a.curr[v];
+        const unsigned long long ckey = ((unsigned long long)j << 32) | (uint32_t)cc;
+        if (threadIdx.x == 0) {
+            // the own community's run, if any neighbour is in it
+            int32_t lo = r0, hi = r1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (keys[starts[mid]] < ckey) lo = mid + 1;
+                else hi = mid;
+            }
+            double self = 0.0, ecc = 0.0;
+            if (lo < r1 && keys[starts[lo]] == ckey) {
+                ecc = hub_run_fold(a.edges, vals, starts[lo], starts[lo + 1], true, v, self);
+            }
+            s_ecc = ecc;
+            s_self = self;
+        }
+        __syncthreads();
+        const double ecc = s_ecc;
+        const double kv = a.vdeg[v];
+        const double eix = ecc - s_self;
+        const double ax = a.cdeg[cc] - kv;
+        double bg = 0.0;
+        int32_t bc = INT_MAX;
+        for (int32_t r = r0 + threadIdx.x; r < r1; r += kHubBlock) {
+            const int64_t b = starts[r], e = starts[r + 1];
+            const int32_t c = (int32_t)(uint32_t)(keys[b] & 0xffffffffull);
+            if ((int64_t)c == cc) continue;
+            double unused = 0.0;
+            const double ey = hub_run_fold(a.edges, vals, b, e, false, v, unused);
+            const double ay = a.cdeg[c];
+            const double gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
+            argmax_take(bg, bc, gain > 0.0 ? gain : 0.0, gain > 0.0 ? c : INT_MAX);
+        }
+        for (int o = 32; o > 0; o >>= 1) argmax_take(bg, bc, __shfl_xor(bg, o), __shfl_xor(bc, o));
+        if ((threadIdx.x & 63) == 0) {
+            s_bg[threadIdx.x / 64] = bg;
+            s_bc[threadIdx.x / 64] = bc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kHubBlock / 64; w++) argmax_take(bg, bc, s_bg[w], s_bc[w]);
+            int64_t tgt = cc;
+            if (bg > 0.0) {
+                tgt = bc;
+                if (a.csize[cc] == 1 && tgt > cc && a.csize[tgt] == 1) tgt = cc;  // louvain.cpp:2275-2276
+            }
+            a.target[v] = tgt;
+            a.cw[v] = ecc;
+            if (a.frozen) a.frozen[v] = ecc;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)inactive);
+}
+
 __global__ void k_hbm_table_init(int32_t* keys, double* vals, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1037,17 +1192,38 @@ __global__ void k_hbm_table_init(int32_t* keys, double* vals, int64_t n)
 // ------------------------------------------------------------------------------------------
 
 // distSumVertexDegree (louvain.cpp:2161-2186) + distInitComm (2564-2581) + ET state.
-__global__ void k_init(const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, int64_t nv,
-                       double* __restrict__ vdeg, double* __restrict__ cdeg, int64_t* __restrict__ csize,
-                       int64_t* __restrict__ past, int64_t* __restrict__ curr, int64_t* __restrict__ target,
-                       unsigned char* __restrict__ active, double* __restrict__ frozen,
+// vDegree: left fold of each row's weights from 0.0 in CSR order (distSumVertexDegree,
+// louvain.cpp:2161-2186), one thread per row.  Eight loads are issued ahead of the dependent
+// adds, so a long row (coarse levels, hubs) streams at memory rate instead of one load
+// latency per edge.
+constexpr int kFoldRows = 256;
+
+__global__ void __launch_bounds__(kFoldRows) k_row_fold(const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
+                                                        int64_t nv, double* __restrict__ vdeg)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = offs[i + 1];
+        int64_t k = offs[i];
+        double acc = 0.0;
+        for (; k + 8 <= e; k += 8) {
+            double w[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) w[u] = edges[k + u].weight;
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = acc + w[u];
+        }
+        for (; k < e; k++) acc = acc + edges[k].weight;
+        vdeg[i] = acc;
+    }
+}
+
+__global__ void k_init(int64_t nv, const double* __restrict__ vdeg, double* __restrict__ cdeg,
+                       int64_t* __restrict__ csize, int64_t* __restrict__ past, int64_t* __restrict__ curr,
+                       int64_t* __restrict__ target, unsigned char* __restrict__ active, double* __restrict__ frozen,
                        double* __restrict__ p_curr, double* __restrict__ p_prev)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
-        double tw = 0.0;
-        for (int64_t k = offs[i]; k < offs[i + 1]; k++) tw = tw + edges[k].weight;
-        vdeg[i] = tw;
-        cdeg[i] = tw;
+        cdeg[i] = vdeg[i];
         csize[i] = 1;
         past[i] = i;
         curr[i] = i;
@@ -1291,6 +1467,7 @@ struct ColorSweep {
     SweepPlan plan{};
     int64_t fused_blocks = 0;
     int64_t mid_off = 0, mid_n = 0, wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0, lpv_off = 0, lpv_n = 0;
+    int32_t index = 0;  // position in Phase::sweeps
 };
 
 // Rows up to this degree take the lane-per-vertex kernel when VITE_AMD_LPV_MAXDEG is set
@@ -1320,6 +1497,10 @@ struct Phase {
     std::vector<int64_t> color_ptr;      // colorPtr (louvain.cpp:647-661)
     int64_t hub_cap = 0;
     int hub_blocks = 0;
+    // Hub rows by sorting (k_hub_*): per color, the row offsets of its hubs in the key array.
+    bool hub_sort = true;
+    std::vector<DBuf<int64_t>> hub_offs;
+    std::vector<int64_t> hub_total;
     // Batch of disjoint graphs (null for one graph): graph of each vertex, per-graph
     // "still iterating" flags and per-graph 1/2m.
     const int32_t* gid = nullptr;
@@ -1380,11 +1561,17 @@ struct Phase {
         ucount = DBuf<unsigned long long>(ctx, 4);
     }
 
+    void init_state(bool with_et)
+    {
+        if (nv) VT_LAUNCH(ctx, "row_fold", k_row_fold, grid_for(nv, kFoldRows, 16384), kFoldRows, 0, g->offs, g->edges, nv, vdeg.p);
+        VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, nv, (const double*)vdeg.p, cdeg.p, csize.p, past.p,
+                  curr.p, target.p, with_et ? active.p : nullptr, with_et ? frozen.p : nullptr,
+                  with_et ? pa.p : nullptr, with_et ? pb.p : nullptr);
+    }
+
     void init(int flags)
     {
-        VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, g->offs, g->edges, nv, vdeg.p, cdeg.p,
-                  csize.p, past.p, curr.p, target.p, et ? active.p : nullptr, et ? frozen.p : nullptr,
-                  et ? pa.p : nullptr, et ? pb.p : nullptr);
+        init_state(et);
         if (flags & VT_FLAG_SEQUENTIAL_SUMS) seq_sum2(ctx, vdeg.p, nullptr, nv, sums());
         else tree_sum2(ctx, vdeg.p, nullptr, nv, sums());
         VT_LAUNCH(ctx, "constant", k_constant, 1, 64, 0, (const double*)sums(), constant());
@@ -1448,9 +1635,27 @@ struct Phase {
             cs.plan.bbeg[kFusedBuckets] = blocks;
             cs.fused_blocks = blocks;
         }
+        for (int64_t c = 0; c < ncol; c++) sweeps[c].index = (int32_t)c;
+        // Hub rows: by sorting, unless the sweep must stay free of library sorts (a batch
+        // replays its sweeps from captured graphs) or edge indexes exceed 32 bits.
+        hub_sort = hub_sort && !gid && g->ne < (int64_t(1) << 32);
+        hub_offs.clear();
+        hub_total.assign(ncol, 0);
+        if (hub_sort) {
+            for (int64_t c = 0; c < ncol; c++) {
+                const ColorSweep& cs = sweeps[c];
+                hub_offs.emplace_back(ctx, (size_t)cs.hub_n + 1);
+                if (!cs.hub_n) continue;
+                DBuf<int64_t> deg(ctx, (size_t)cs.hub_n + 1);
+                VT_LAUNCH(ctx, "hub_degrees", k_hub_degrees, grid_for(cs.hub_n + 1, 256, 1024), 256, 0,
+                          (const int32_t*)(lists.p + cs.hub_off), (int32_t)cs.hub_n, (const int64_t*)g->offs, deg.p);
+                exclusive_scan_i64(ctx, deg.p, hub_offs.back().p, cs.hub_n + 1);
+                d2h(ctx, &hub_total[c], hub_offs.back().p + cs.hub_n, 8);
+            }
+        }
         // Hub tables: one HBM table per resident wave, sized for the largest row.
         hub_cap = 0;
-        if (max_hub > 0) {
+        if (max_hub > 0 && !hub_sort) {
             int64_t cap = 1;
             while (cap < 2 * (g->max_degree + 1)) cap <<= 1;
             hub_cap = cap;
@@ -1538,7 +1743,32 @@ struct Phase {
         if (cs.wide_n)
             VT_LAUNCH(ctx, "local_move_wide", k_local_move_wide, (unsigned)std::min<int64_t>(cs.wide_n, 8192), 64, 0, a,
                       (const int32_t*)(lists.p + cs.wide_off), cs.wide_n);
-        if (cs.hub_n) {
+        if (cs.hub_n && hub_sort) {
+            const int64_t total = hub_total[cs.index];
+            const int32_t nh = (int32_t)cs.hub_n;
+            const int32_t* hubs = (const int32_t*)(lists.p + cs.hub_off);
+            const int64_t* hoffs = hub_offs[cs.index].p;
+            const size_t t1 = (size_t)std::max<int64_t>(total, 1);
+            unsigned long long* k0 = (unsigned long long*)scratch(ctx, "hs_k0", 8 * t1);
+            unsigned long long* k1 = (unsigned long long*)scratch(ctx, "hs_k1", 8 * t1);
+            uint32_t* v0 = (uint32_t*)scratch(ctx, "hs_v0", 4 * t1);
+            uint32_t* v1 = (uint32_t*)scratch(ctx, "hs_v1", 4 * t1);
+            int32_t* head = (int32_t*)scratch(ctx, "hs_head", 4 * t1);
+            int32_t* pos = (int32_t*)scratch(ctx, "hs_pos", 4 * t1);
+            int32_t* starts = (int32_t*)scratch(ctx, "hs_starts", 4 * t1 + 4);
+            VT_LAUNCH(ctx, "hub_keys", k_hub_keys, (unsigned)std::min<int64_t>(nh, 16384), 256, 0, hubs, nh, hoffs,
+                      (const int64_t*)g->offs, (const DEdge*)g->edges, (const int64_t*)curr.p, k0, v0);
+            bool alt = false;
+            sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, 32 + (int)bits_for(nh), (uint64_t*)k1, v1, &alt);
+            const unsigned long long* ks = alt ? k1 : k0;
+            const uint32_t* vs = alt ? v1 : v0;
+            VT_LAUNCH(ctx, "hub_heads", k_hub_heads, grid_for(total, 256, 16384), 256, 0, ks, total, head);
+            exclusive_scan_i32(ctx, head, pos, total);
+            VT_LAUNCH(ctx, "hub_starts", k_hub_starts, grid_for(total, 256, 16384), 256, 0, (const int32_t*)head,
+                      (const int32_t*)pos, total, starts);
+            VT_LAUNCH(ctx, "local_move_hub", k_hub_move, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs, nh,
+                      hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)starts);
+        } else if (cs.hub_n) {
             const int blocks = (int)std::min<int64_t>(cs.hub_n, hub_blocks);
             VT_LAUNCH(ctx, "local_move_hub", k_move_hbm, (unsigned)blocks, 64, 0, a, (const int32_t*)(lists.p + cs.hub_off),
                       cs.hub_n, (int32_t*)scratch(ctx, "hub_keys", 0), (double*)scratch(ctx, "hub_vals", 0),
@@ -1754,10 +1984,10 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     ph.gid = d_gid;
     ph.gact = gact.p;
     ph.gconst = gconst.p;
+    ph.hub_sort = false;  // the sweep is replayed from a captured graph: no library sort inside
     const bool seq = (flags & VT_FLAG_SEQUENTIAL_SUMS) != 0;
 
-    VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, u->offs, u->edges, nv, ph.vdeg.p, ph.cdeg.p,
-              ph.csize.p, ph.past.p, ph.curr.p, ph.target.p, nullptr, nullptr, nullptr, nullptr);
+    ph.init_state(false);
     if (seq) seg_seq_sum2(ctx, seg, ph.vdeg.p, nullptr, gsums.p);
     else seg_tree_sum2(ctx, seg, ph.vdeg.p, nullptr, gsums.p);
     VT_LAUNCH(ctx, "seg_constant", k_seg_constant, grid_for(ng, 256, 64), 256, 0, (const double*)gsums.p, ng, gconst.p);

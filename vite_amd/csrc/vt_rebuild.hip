@@ -6,9 +6,9 @@
 //  2. one (C1,C2) key per fine edge with the edge's weight, radix-sorted stably, so equal
 //     keys keep global edge order = "vertices of C1 ascending, each row in CSR order"
 //     (fill_newEdgesMap, rebuild.cpp:294-329);
-//  3. the head of every key run folds its weights left to right, the first contribution
-//     stored as is; runs come out sorted by C2 inside C1 (std::map order,
-//     rebuild.cpp:469-476).  Self-loops are kept.
+//  3. every key run is folded left to right by one thread, the first contribution stored
+//     as is; runs come out sorted by C2 inside C1 (std::map order, rebuild.cpp:469-476).
+//     Self-loops are kept.
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -55,18 +55,41 @@ __global__ void k_total(const int64_t* __restrict__ flag, const int64_t* __restr
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = n ? scan[n - 1] + flag[n - 1] : 0;
 }
 
-// One key per fine edge: (C1 << bits) | C2, value = weight.
-__global__ void k_edge_keys(const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, int64_t nv,
-                            const uint32_t* __restrict__ newid, int bits, unsigned long long* __restrict__ keys,
-                            double* __restrict__ vals)
+// One key per fine edge: (C1 << bits) | C2, value = weight.  A block takes 256 consecutive
+// rows, caches their starts and new ids in LDS, and walks their (contiguous) edges with
+// consecutive threads on consecutive edges; each thread finds its edge's row by binary
+// search over the cached starts.
+constexpr int kRowsPerBlock = 256;
+
+__global__ void __launch_bounds__(kRowsPerBlock) k_edge_keys(const int64_t* __restrict__ offs,
+                                                             const DEdge* __restrict__ edges, int64_t nv,
+                                                             const uint32_t* __restrict__ newid, int bits,
+                                                             unsigned long long* __restrict__ keys,
+                                                             double* __restrict__ vals)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long c1 = (unsigned long long)newid[i] << bits;
-        for (int64_t k = offs[i]; k < offs[i + 1]; k++) {
+    __shared__ int64_t so[kRowsPerBlock + 1];
+    __shared__ uint32_t sid[kRowsPerBlock];
+    for (int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock; r0 < nv; r0 += (int64_t)gridDim.x * kRowsPerBlock) {
+        const int nr = (int)(nv - r0 < kRowsPerBlock ? nv - r0 : kRowsPerBlock);
+        if ((int)threadIdx.x < nr) {
+            so[threadIdx.x] = offs[r0 + threadIdx.x];
+            sid[threadIdx.x] = newid[r0 + threadIdx.x];
+        }
+        if (threadIdx.x == 0) so[nr] = offs[r0 + nr];
+        __syncthreads();
+        const int64_t e1 = so[nr];
+        for (int64_t k = so[0] + threadIdx.x; k < e1; k += kRowsPerBlock) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (so[mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
             const DEdge e = edges[k];
-            keys[k] = c1 | newid[e.tail];
+            keys[k] = ((unsigned long long)sid[lo] << bits) | newid[e.tail];
             vals[k] = e.weight;
         }
+        __syncthreads();
     }
 }
 
@@ -76,21 +99,39 @@ __global__ void k_run_heads(const unsigned long long* __restrict__ keys, int64_t
         head[j] = (j == 0 || keys[j] != keys[j - 1]) ? 1 : 0;
 }
 
-// Head of each run: left fold of the run's weights, written at its output position.
-__global__ void k_run_fold(const unsigned long long* __restrict__ keys, const double* __restrict__ vals, int64_t n,
-                           const int64_t* __restrict__ head, const int64_t* __restrict__ pos, int bits,
-                           DEdge* __restrict__ out, unsigned long long* __restrict__ rowcount)
+// starts[r] = first position of run r (pos = exclusive scan of the head flags).
+__global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __restrict__ pos, int64_t n, int64_t nu,
+                             int64_t* __restrict__ starts)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        if (head[j]) starts[pos[j]] = j;
+    if (blockIdx.x == 0 && threadIdx.x == 0) starts[nu] = n;
+}
+
+// Run r: left fold of its weights in sorted (= fine edge) order, the first contribution as
+// is; the bounds are known up front, so the loads run ahead of the dependent adds.
+__global__ void k_run_fold(const unsigned long long* __restrict__ keys, const double* __restrict__ vals,
+                           const int64_t* __restrict__ starts, int64_t nu, int bits, DEdge* __restrict__ out,
+                           unsigned long long* __restrict__ rowcount)
 {
     const unsigned long long lomask = (1ull << bits) - 1;
-    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        if (!head[j]) continue;
-        const unsigned long long key = keys[j];
-        double acc = vals[j];
-        for (int64_t k = j + 1; k < n && keys[k] == key; k++) acc = acc + vals[k];
-        DEdge e;
-        e.tail = (int64_t)(key & lomask);
-        e.weight = acc;
-        out[pos[j]] = e;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nu; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = starts[r], e = starts[r + 1];
+        const unsigned long long key = keys[b];
+        double acc = vals[b];
+        int64_t k = b + 1;
+        for (; k + 4 <= e; k += 4) {
+            const double x0 = vals[k], x1 = vals[k + 1], x2 = vals[k + 2], x3 = vals[k + 3];
+            acc = acc + x0;
+            acc = acc + x1;
+            acc = acc + x2;
+            acc = acc + x3;
+        }
+        for (; k < e; k++) acc = acc + vals[k];
+        DEdge o;
+        o.tail = (int64_t)(key & lomask);
+        o.weight = acc;
+        out[r] = o;
         atomicAdd(&rowcount[key >> bits], 1ull);
     }
 }
@@ -126,8 +167,8 @@ vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect)
     DBuf<unsigned long long> k0(ctx, e1), k1(ctx, e1);
     DBuf<double> v0(ctx, e1), v1(ctx, e1);
     if (nv)
-        VT_LAUNCH(ctx, "rb_keys", k_edge_keys, grid_for(nv, 256, 16384), 256, 0, g->offs, g->edges, nv,
-                  (const uint32_t*)newid.p, bits, k0.p, v0.p);
+        VT_LAUNCH(ctx, "rb_keys", k_edge_keys, grid_for(nv, kRowsPerBlock, 16384), kRowsPerBlock, 0, g->offs, g->edges,
+                  nv, (const uint32_t*)newid.p, bits, k0.p, v0.p);
     const unsigned long long* ks = k0.p;
     const double* vs = v0.p;
     if (ne > 1) {
@@ -155,9 +196,13 @@ vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect)
     try {
         DBuf<unsigned long long> rc(ctx, (size_t)nc + 1);
         dzero(ctx, rc.p, 8 * ((size_t)nc + 1));
-        if (ne)
-            VT_LAUNCH(ctx, "rb_fold", k_run_fold, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, (const int64_t*)head.p,
-                      (const int64_t*)pos.p, bits, out->edges, rc.p);
+        if (ne) {
+            DBuf<int64_t> starts(ctx, (size_t)nu + 1);
+            VT_LAUNCH(ctx, "rb_starts", k_run_starts, grid_for(ne, 256, 16384), 256, 0, (const int64_t*)head.p,
+                      (const int64_t*)pos.p, ne, nu, starts.p);
+            VT_LAUNCH(ctx, "rb_fold", k_run_fold, grid_for(nu, 256, 16384), 256, 0, ks, vs, (const int64_t*)starts.p, nu,
+                      bits, out->edges, rc.p);
+        }
         exclusive_scan_i64(ctx, (const int64_t*)rc.p, out->offs, nc + 1);
         graph_finalize(ctx, out);
     } catch (...) {
