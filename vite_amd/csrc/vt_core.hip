@@ -514,6 +514,7 @@ int vt_ctx_create(int device, uint64_t hbm_budget_bytes, vt_ctx** out)
     vt_ctx* c = new vt_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
+    c->tuning = vt::default_tuning();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return VT_ERR_HIP;

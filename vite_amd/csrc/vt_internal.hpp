@@ -58,7 +58,7 @@ struct vt_ctx {
     // pinned host staging for small readbacks
     void* pinned = nullptr;
     // sweep launch shape (performance only: every setting gives the same results)
-    vt_tuning tuning = {{7, 15, 31, 63, 127, 1023}, 16384};
+    vt_tuning tuning = {{7, 15, 31, 63, 127, 1023}, 16384};  // replaced by default_tuning() at creation
 };
 
 struct vt_graph {
@@ -192,6 +192,7 @@ void graph_finalize(vt_ctx* ctx, vt_graph* g);  // computes max_degree
 void graph_delete(vt_ctx* ctx, vt_graph* g);
 
 int64_t bits_for(int64_t maxval);
+vt_tuning default_tuning();  // bucket bounds at each kernel's capacity (vt_louvain.hip)
 
 // ---- batch of disjoint graphs (vt_louvain.hip) ----------------------------------------------
 void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,

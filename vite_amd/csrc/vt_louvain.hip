@@ -308,6 +308,20 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 
 // Insert-or-find without an occupancy list: CAS first (one LDS round trip whether the key
 // is new or already present).
+// Edge chunks per row for the 8- and 16-lane groups (the others take two): a chunk pass
+// costs a wave about the same instructions whatever G is, so narrower groups with more
+// chunks serve more vertices per wave.
+#ifndef VT_NCH_G8
+#define VT_NCH_G8 2
+#endif
+#ifndef VT_NCH_G16
+#define VT_NCH_G16 3
+#endif
+constexpr int kNch[5] = {2, VT_NCH_G8, VT_NCH_G16, 2, 2};
+// Largest row each bucket's kernel takes: NCH * G - 1 (G = 4, 8, 16, 32, 64), then the
+// one-wave 2048-slot kernel.
+constexpr int32_t kBucketCap[6] = {2 * 4 - 1, VT_NCH_G8 * 8 - 1, VT_NCH_G16 * 16 - 1, 2 * 32 - 1, 2 * 64 - 1, 1023};
+
 #ifndef VT_SPLIT_EDGES
 #define VT_SPLIT_EDGES 1
 #endif
@@ -502,77 +516,68 @@ __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t
 #endif
 constexpr bool kPipeRows = VT_PIPE_ROWS;
 
-// Rows of degree <= 2G - 1 (every fused bucket), table CAP = 4G slots, software-pipelined
-// across the vertices a group processes: while vertex i is folded in LDS and its candidates'
-// community degrees are gathered, vertex i+1's two edge chunks and neighbour-community
-// gathers are already in flight.  Same results as move_vertex.
+// Rows of degree <= NCH*G - 1 read in NCH chunks of G edges, table CAP = 4G slots,
+// software-pipelined across the vertices a group processes: while vertex i is folded in
+// LDS and its candidates' community degrees are gathered, vertex i+1's edge chunks and
+// neighbour-community gathers are already in flight.  Same results as move_vertex.
 
+template <int NCH>
 struct RowRegs {
-    int32_t t0, t1;   // neighbour ids (low words; ids < 2^31)
-    double w0, w1;
-    int32_t c0, c1;   // their communities
-    bool ok0, ok1, self0, self1;
+    int32_t t[NCH];   // neighbour ids (low words; ids < 2^31)
+    double w[NCH];
+    int32_t c[NCH];   // their communities
+    bool ok[NCH], self[NCH];
 };
 
-// Edge records are {int64 tail, f64 weight}; only the tail's low word and the weight are
-// read, which keeps a chunk in flight at three registers a lane.
-template <int G, bool BATCH>
-__device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader<BATCH>& h, int lane, RowRegs& r)
+// Only the tail (32 bits) and the weight of an edge are read, which keeps a chunk in
+// flight at three registers a lane.
+template <int G, int NCH, bool BATCH>
+__device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader<BATCH>& h, int lane,
+                                                RowRegs<NCH>& r)
 {
-    const int64_t j0 = h.e0 + lane, j1 = h.e0 + G + lane;
-    r.ok0 = lane < h.deg;
-    r.ok1 = G + lane < h.deg;
-    r.t0 = r.t1 = -1;
-    r.w0 = r.w1 = 0.0;
-    if constexpr (kSplitEdges) {
-        if (r.ok0) {
-            r.t0 = a.tail32[j0];
-            r.w0 = a.wgt[j0];
-        }
-        if (r.ok1) {
-            r.t1 = a.tail32[j1];
-            r.w1 = a.wgt[j1];
-        }
-    } else {
-        const int32_t* words = reinterpret_cast<const int32_t*>(a.edges);
-        const double* wts = reinterpret_cast<const double*>(a.edges);
-        if (r.ok0) {
-            r.t0 = words[4 * j0];
-            r.w0 = wts[2 * j0 + 1];
-        }
-        if (r.ok1) {
-            r.t1 = words[4 * j1];
-            r.w1 = wts[2 * j1 + 1];
+#pragma unroll
+    for (int u = 0; u < NCH; u++) {
+        const int64_t j = h.e0 + u * G + lane;
+        r.ok[u] = u * G + lane < h.deg;
+        r.t[u] = -1;
+        r.w[u] = 0.0;
+        if (r.ok[u]) {
+            if constexpr (kSplitEdges) {
+                r.t[u] = a.tail32[j];
+                r.w[u] = a.wgt[j];
+            } else {
+                r.t[u] = reinterpret_cast<const int32_t*>(a.edges)[4 * j];
+                r.w[u] = reinterpret_cast<const double*>(a.edges)[2 * j + 1];
+            }
         }
     }
 }
 
-__device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, RowRegs& r)
+template <int NCH>
+__device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, RowRegs<NCH>& r)
 {
-    r.c0 = 0;
-    r.c1 = 0;
-    if constexpr (kSplitEdges) {
-        if (r.ok0) r.c0 = a.curr32[r.t0];
-        if (r.ok1) r.c1 = a.curr32[r.t1];
-    } else {
-        const int32_t* cw = reinterpret_cast<const int32_t*>(a.curr);
-        if (r.ok0) r.c0 = cw[2 * (int64_t)r.t0];
-        if (r.ok1) r.c1 = cw[2 * (int64_t)r.t1];
+#pragma unroll
+    for (int u = 0; u < NCH; u++) {
+        r.c[u] = 0;
+        if (r.ok[u]) {
+            if constexpr (kSplitEdges) r.c[u] = a.curr32[r.t[u]];
+            else r.c[u] = reinterpret_cast<const int32_t*>(a.curr)[2 * (int64_t)r.t[u]];
+        }
+        r.self[u] = r.ok[u] && r.t[u] == v;
     }
-    r.self0 = r.ok0 && r.t0 == v;
-    r.self1 = r.ok1 && r.t1 == v;
 }
 
-template <int G, int CAP, int BLOCK, bool PIPE, bool BATCH>
+template <int G, int CAP, int BLOCK, bool PIPE, bool BATCH, int NCH>
 __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const GroupTable<G>& t, double* ecc_cell,
                                                   const int32_t* __restrict__ list, int64_t n, int64_t idx,
                                                   int64_t stride, int lane, int gbase, unsigned* inactive_local)
 {
     static_assert(CAP == 4 * G && CAP <= 256, "occ holds 8-bit slot indexes");
+    static_assert(NCH * G <= 3 * CAP / 4, "at most 3/4 of the table filled by distinct communities");
     if (idx >= n) return;
     VHeader<BATCH> h = load_header<BATCH>(a, list[idx], true);
     int32_t vn = (idx + stride < n) ? list[idx + stride] : 0;
-    RowRegs r;
+    RowRegs<NCH> r;
     if constexpr (PIPE) row_issue_edges<G>(a, h, lane, r);
     VHeader<BATCH> hn = load_header<BATCH>(a, vn, idx + stride < n);
     if constexpr (PIPE) row_issue_gathers(a, h.v, r);
@@ -582,7 +587,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
         // list entry two ahead, header one ahead (and with PIPE the next vertex's edges)
         const int64_t i2 = idx + 2 * stride;
         const int32_t vnn = (i2 < n) ? list[i2] : 0;
-        RowRegs rn;
+        RowRegs<NCH> rn;
         if constexpr (PIPE) {
             row_issue_edges<G>(a, hn, lane, rn);
         } else {
@@ -614,9 +619,12 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             if (lane == 0) *ecc_cell = 0.0;
             double selfLoop = 0.0;
             int nocc = 0;
-            fold_chunk<G>(t, ecc_cell, r.c0, (int32_t)cc, r.w0, r.ok0, r.self0, lane, gbase, selfLoop, nocc);
-            if (h.deg > G)
-                fold_chunk<G>(t, ecc_cell, r.c1, (int32_t)cc, r.w1, r.ok1, r.self1, lane, gbase, selfLoop, nocc);
+            fold_chunk<G>(t, ecc_cell, r.c[0], (int32_t)cc, r.w[0], r.ok[0], r.self[0], lane, gbase, selfLoop, nocc);
+#pragma unroll
+            for (int u = 1; u < NCH; u++)
+                if (h.deg > u * G)
+                    fold_chunk<G>(t, ecc_cell, r.c[u], (int32_t)cc, r.w[u], r.ok[u], r.self[u], lane, gbase, selfLoop,
+                                  nocc);
             const double ecc = *ecc_cell;
             // the next vertex's neighbour communities, before waiting on this one's gathers
             if (has_next) row_issue_gathers(a, hn.v, rn);
@@ -700,7 +708,7 @@ constexpr bool kXcdRanges = VT_XCD_RANGES;
 constexpr int kXcds = 8;
 
 // LDS carve-out of the two-chunk group buckets.
-template <int G, int CAP, int BLOCK>
+template <int G, int CAP, int BLOCK, int NCH = 2>
 struct GroupLds {
     using mask_t = typename MaskT<G>::type;
     static constexpr int NG = BLOCK / G;
@@ -711,12 +719,12 @@ struct GroupLds {
     static constexpr size_t ecc = align16(wst + sizeof(double) * NG * (G + 1));
     static constexpr size_t trash = align16(ecc + sizeof(double) * NG);  // val, mask, key, occ
     static constexpr size_t occ = trash + 32;
-    static constexpr size_t bytes = align16(occ + (size_t)NG * 2 * G);
+    static constexpr size_t bytes = align16(occ + (size_t)NG * NCH * G);
 };
 
 // Runs one degree bucket inside a block: block `blk` of `nblk` owns list entries
 // blk*NG + g, stepping by nblk*NG.  Every thread of the block must call it.
-template <int G, int CAP, int BLOCK, bool TWO, bool BATCH = false>
+template <int G, int CAP, int BLOCK, bool TWO, bool BATCH = false, int NCH = 2>
 __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const int32_t* __restrict__ list, int64_t n,
                                            int64_t blk, int64_t nblk, unsigned* inactive)
 {
@@ -726,7 +734,7 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
     const int64_t stride = nblk * (BLOCK / G);
     int64_t idx = blk * (BLOCK / G) + g;
     if constexpr (TWO) {
-        using L = GroupLds<G, CAP, BLOCK>;
+        using L = GroupLds<G, CAP, BLOCK, NCH>;
         using mask_t = typename L::mask_t;
         constexpr int NG = L::NG;
         int32_t* keys = (int32_t*)(lds + L::keys);
@@ -741,7 +749,7 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
         if (threadIdx.x < NG) wst[threadIdx.x * (G + 1)] = -0.0;
         __syncthreads();
         char* tr = lds + L::trash;
-        GroupTable<G> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, (uint8_t*)(lds + L::occ) + g * 2 * G,
+        GroupTable<G> t{keys + g * CAP, vals + g * CAP, masks + g * CAP, (uint8_t*)(lds + L::occ) + g * NCH * G,
                         wst + g * (G + 1) + 1, (uint32_t)(CAP - 1), (int32_t*)(tr + 16), (double*)tr,
                         (mask_t*)(tr + 8), (uint8_t*)(tr + 20)};
         double* ecc_cell = (double*)(lds + L::ecc) + g;
@@ -751,12 +759,12 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
             // contiguous eighth of the list: neighbouring vertices then share one L2.
             const int64_t x = blk % kXcds, per = (n + kXcds - 1) / kXcds;
             const int64_t lo = x * per < n ? x * per : n, hi = lo + per < n ? lo + per : n;
-            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH>(a, t, ecc_cell, list + lo, hi - lo,
+            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH, NCH>(a, t, ecc_cell, list + lo, hi - lo,
                                                                (blk / kXcds) * (BLOCK / G) + g,
                                                                (nblk / kXcds) * (BLOCK / G), lane, gbase, inactive);
         } else {
-            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH>(a, t, ecc_cell, list, n, idx, stride, lane, gbase,
-                                                               inactive);
+            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH, NCH>(a, t, ecc_cell, list, n, idx, stride, lane, gbase,
+                                                                    inactive);
         }
     } else {
         using L = LdsLayout<G, CAP, BLOCK>;
@@ -787,8 +795,8 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
 // that its larger tables do not set the LDS footprint, and so the occupancy, of this one.
 constexpr int kFusedBuckets = 4;
 constexpr int kFusedBlock = 256;
-constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256>::bytes, GroupLds<8, 32, 256>::bytes,
-                                       GroupLds<16, 64, 256>::bytes, GroupLds<32, 128, 256>::bytes});
+constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256, kNch[0]>::bytes, GroupLds<8, 32, 256, kNch[1]>::bytes,
+                                       GroupLds<16, 64, 256, kNch[2]>::bytes, GroupLds<32, 128, 256, kNch[3]>::bytes});
 constexpr size_t kMidLds = GroupLds<64, 256, 256>::bytes;
 static_assert(kFusedLds <= 20480, "eight 256-lane blocks per CU");
 
@@ -813,10 +821,10 @@ k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
     const int64_t blk = blockIdx.x - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
     const int32_t* list = lists + plan.off[b];
     switch (b) {
-        case 0: run_bucket<4, 16, 256, true, BATCH>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
-        case 1: run_bucket<8, 32, 256, true, BATCH>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
-        case 2: run_bucket<16, 64, 256, true, BATCH>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
-        default: run_bucket<32, 128, 256, true, BATCH>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+        case 0: run_bucket<4, 16, 256, true, BATCH, kNch[0]>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
+        case 1: run_bucket<8, 32, 256, true, BATCH, kNch[1]>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+        case 2: run_bucket<16, 64, 256, true, BATCH, kNch[2]>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
+        default: run_bucket<32, 128, 256, true, BATCH, kNch[3]>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
     }
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
@@ -829,7 +837,7 @@ __global__ void __launch_bounds__(256) k_local_move_mid(MoveArgs a, const int32_
     __shared__ __align__(16) char lds[kMidLds];
     __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
-    run_bucket<64, 256, 256, true, BATCH>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
+    run_bucket<64, 256, 256, true, BATCH, kNch[4]>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
@@ -2090,6 +2098,14 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     if (rounds_out) *rounds_out = rounds;
 }
 
+vt_tuning default_tuning()
+{
+    vt_tuning t;
+    for (int b = 0; b < 6; b++) t.bucket_max_degree[b] = kBucketCap[b];
+    t.blocks_per_bucket = 16384;
+    return t;
+}
+
 }  // namespace vt
 
 using namespace vt;
@@ -2145,7 +2161,7 @@ int vt_phase_sweep(vt_ctx* ctx, vt_phase* st)
 static void check_tuning(vt_ctx* ctx, const vt_tuning* t)
 {
     // a row in bucket b has degree <= bound[b]; the kernel of bucket b handles up to cap[b]
-    static const int32_t cap[6] = {7, 15, 31, 63, 127, 1023};
+    const int32_t* cap = kBucketCap;
     for (int b = 0; b < 6; b++)
         if (t->bucket_max_degree[b] < -1 || t->bucket_max_degree[b] > cap[b])
             fail(ctx, VT_ERR_ARG, "vt_tuning: bucket_max_degree[%d] = %d outside [-1, %d]", b, t->bucket_max_degree[b],
@@ -2207,7 +2223,9 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
             double tb = time_it(best);
             for (int b = 0; b < 5; b++) {
                 const int G = 4 << b;
-                for (int cand : {G - 1, (3 * G) / 2 - 1, 2 * G - 1}) {
+                std::vector<int> cands;
+                for (int h = 2; h <= 2 * kNch[b]; h++) cands.push_back(h * G / 2 - 1);  // 1, 1.5, 2, ... chunks
+                for (int cand : cands) {
                     if (cand == best.bucket_max_degree[b]) continue;
                     vt_tuning t = best;
                     t.bucket_max_degree[b] = cand;
