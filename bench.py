@@ -276,6 +276,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttq", action="store_true", help="skip the time-to-final-Q run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batched-form run (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=256,
                     help="graphs of the batch also run one graph per launch (the comparison)")
@@ -315,6 +316,11 @@ def main():
     V, E = dg.nv, dg.ne
     log(f"graph: n={V} directed entries={E} generated in {gen_s:.2f}s")
 
+    tuning = ctx.tuning
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = ctx.autotune(dg, prep=args.prep, reps=3)
+        log(f"autotune: {tuning} in {time.perf_counter() - t0:.2f}s")
     st = dg.phase_begin(lower=-1.0, thresh=1e-3)
     for _ in range(args.prep):
         q, done = st.step()
@@ -353,7 +359,7 @@ def main():
         "config": {"workload": f"RGG n={V} r=sqrt(2 ln n/(pi n)) +{args.extra}% extra edges U(0,1), "
                                f"local-move sweep after {args.prep} phase-0 iterations",
                    "n": V, "directed_entries": E, "seed": args.seed, "ids": "int64", "weights": "f64",
-                   "threshold": 1e-3},
+                   "threshold": 1e-3, "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "local_move", "kernel_avg_ms": kavg, "algorithmic_bytes_per_launch": alg_bytes,

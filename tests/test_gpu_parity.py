@@ -243,7 +243,7 @@ def test_tuning_does_not_change_results(ctx, name, tuning):
 
 def test_tuning_rejects_bounds_beyond_kernel_capacity(ctx):
     default = ctx.tuning
-    for bad in [[8, 15, 31, 63, 127, 1023], [7, 15, 31, 64, 127, 1023], [7, 15, 31, 63, 127, 1024]]:
+    for bad in [[8, 15, 31, 63, 127, 1023], [7, 15, 31, 200, 127, 1023], [7, 15, 31, 63, 127, 1024]]:
         with pytest.raises(vite_amd.VtError):
             ctx.set_tuning(bucket_max_degree=bad)
     with pytest.raises(vite_amd.VtError):

@@ -317,10 +317,17 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 #ifndef VT_NCH_G16
 #define VT_NCH_G16 3
 #endif
-constexpr int kNch[5] = {2, VT_NCH_G8, VT_NCH_G16, 2, 2};
+#ifndef VT_NCH_G32
+#define VT_NCH_G32 2
+#endif
+#ifndef VT_NCH_G64
+#define VT_NCH_G64 2
+#endif
+constexpr int kNch[5] = {2, VT_NCH_G8, VT_NCH_G16, VT_NCH_G32, VT_NCH_G64};
 // Largest row each bucket's kernel takes: NCH * G - 1 (G = 4, 8, 16, 32, 64), then the
 // one-wave 2048-slot kernel.
-constexpr int32_t kBucketCap[6] = {2 * 4 - 1, VT_NCH_G8 * 8 - 1, VT_NCH_G16 * 16 - 1, 2 * 32 - 1, 2 * 64 - 1, 1023};
+constexpr int32_t kBucketCap[6] = {2 * 4 - 1, VT_NCH_G8 * 8 - 1, VT_NCH_G16 * 16 - 1, VT_NCH_G32 * 32 - 1,
+                                   VT_NCH_G64 * 64 - 1, 1023};
 
 #ifndef VT_SPLIT_EDGES
 #define VT_SPLIT_EDGES 1
@@ -797,7 +804,7 @@ constexpr int kFusedBuckets = 4;
 constexpr int kFusedBlock = 256;
 constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256, kNch[0]>::bytes, GroupLds<8, 32, 256, kNch[1]>::bytes,
                                        GroupLds<16, 64, 256, kNch[2]>::bytes, GroupLds<32, 128, 256, kNch[3]>::bytes});
-constexpr size_t kMidLds = GroupLds<64, 256, 256>::bytes;
+constexpr size_t kMidLds = GroupLds<64, 256, 256, kNch[4]>::bytes;
 static_assert(kFusedLds <= 20480, "eight 256-lane blocks per CU");
 
 struct SweepPlan {
