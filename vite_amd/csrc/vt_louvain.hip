@@ -519,7 +519,7 @@ __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t
 }
 
 #ifndef VT_PIPE_ROWS
-#define VT_PIPE_ROWS 1
+#define VT_PIPE_ROWS 0
 #endif
 constexpr bool kPipeRows = VT_PIPE_ROWS;
 
