@@ -321,7 +321,7 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 #define VT_NCH_G32 2
 #endif
 #ifndef VT_NCH_G64
-#define VT_NCH_G64 2
+#define VT_NCH_G64 3
 #endif
 constexpr int kNch[5] = {2, VT_NCH_G8, VT_NCH_G16, VT_NCH_G32, VT_NCH_G64};
 // Largest row each bucket's kernel takes: NCH * G - 1 (G = 4, 8, 16, 32, 64), then the
