@@ -54,7 +54,9 @@ typedef struct {
 int vt_ctx_create(int device, uint64_t hbm_budget_bytes, vt_ctx** out);
 int vt_ctx_destroy(vt_ctx* ctx);
 const char* vt_last_error(const vt_ctx* ctx);
-/* Bytes currently allocated through this context (graphs + working state). */
+/* Bytes of live device allocations of this context (graphs + working state); freed blocks
+ * the context keeps for reuse are not counted, and are returned to the driver when the
+ * budget or the device runs short and at vt_ctx_destroy. */
 uint64_t vt_ctx_bytes_in_use(const vt_ctx* ctx);
 /* Per-kernel timing with HIP events on the context stream (0 = off). */
 int vt_ctx_set_profiling(vt_ctx* ctx, int on);

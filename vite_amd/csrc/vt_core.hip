@@ -29,26 +29,59 @@ void hip_check(vt_ctx* ctx, hipError_t e, const char* what, const char* file, in
     if (e != hipSuccess) fail(ctx, VT_ERR_HIP, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
 
+// Device memory: stream-ordered blocks (hipMallocAsync on the context stream) behind a
+// per-context cache.  A freed block is kept and handed to a later request of up to 4x less
+// (all work of a context is ordered on its one stream, so the reuse is safe); freeing a
+// large block back to the driver costs a quarter of a millisecond, and the phase loop frees
+// and reallocates its buffers level after level.  The cache is trimmed when the budget or
+// the device runs short, and released with the scratch buffers.
+void trim_cache(vt_ctx* ctx)
+{
+    for (auto& kv : ctx->cache) (void)hipFreeAsync(kv.second, ctx->stream);
+    ctx->cache.clear();
+    ctx->cached = 0;
+}
+
 void* dalloc(vt_ctx* ctx, size_t bytes)
 {
+    if (ctx->budget && ctx->in_use + ctx->cached + bytes > ctx->budget) trim_cache(ctx);
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
         fail(ctx, VT_ERR_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu in use of %llu)",
              bytes, (unsigned long long)ctx->in_use, (unsigned long long)ctx->budget);
-    // Stream-ordered allocation: unlike hipFree, hipFreeAsync does not wait for the whole
-    // device, so contexts on other streams (the serving slots) keep running.
     void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
-    if (e != hipSuccess) fail(ctx, VT_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
-    ctx->in_use += bytes;
-    if (ctx->in_use > ctx->peak) ctx->peak = ctx->in_use;
+    size_t size = bytes;
+    auto it = ctx->cache.lower_bound(bytes);
+    if (it != ctx->cache.end() && it->first <= std::max<size_t>(4 * bytes, bytes + (1u << 20)) &&
+        (!ctx->budget || ctx->in_use + it->first <= ctx->budget)) {
+        size = it->first;
+        p = it->second;
+        ctx->cache.erase(it);
+        ctx->cached -= size;
+    } else {
+        hipError_t e = hipMallocAsync(&p, bytes, ctx->stream);
+        if (e != hipSuccess && !ctx->cache.empty()) {
+            trim_cache(ctx);
+            (void)hipStreamSynchronize(ctx->stream);
+            e = hipMallocAsync(&p, bytes, ctx->stream);
+        }
+        if (e != hipSuccess) fail(ctx, VT_ERR_HIP, "hipMallocAsync(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    ctx->blocks[p] = size;
+    ctx->in_use += size;
+    if (ctx->in_use + ctx->cached > ctx->peak) ctx->peak = ctx->in_use + ctx->cached;
     return p;
 }
 
-void dfree(vt_ctx* ctx, void* p, size_t bytes)
+void dfree(vt_ctx* ctx, void* p, size_t)
 {
     if (!p) return;
-    (void)hipFreeAsync(p, ctx->stream);
-    ctx->in_use -= std::min<uint64_t>(ctx->in_use, bytes);
+    auto it = ctx->blocks.find(p);
+    if (it == ctx->blocks.end()) return;
+    const size_t size = it->second;
+    ctx->blocks.erase(it);
+    ctx->in_use -= std::min<uint64_t>(ctx->in_use, size);
+    ctx->cache.emplace(size, p);
+    ctx->cached += size;
 }
 
 void* scratch(vt_ctx* ctx, const std::string& key, size_t bytes)
@@ -70,6 +103,7 @@ void release_scratch(vt_ctx* ctx)
 {
     for (auto& kv : ctx->scratch) dfree(ctx, kv.second.first, kv.second.second);
     ctx->scratch.clear();
+    trim_cache(ctx);
 }
 
 TimedScope::TimedScope(vt_ctx* c, const char* n) : ctx(c), name(n)

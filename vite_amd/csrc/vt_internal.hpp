@@ -43,7 +43,10 @@ struct vt_ctx {
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t in_use = 0;
-    uint64_t peak = 0;  // high-water mark of in_use since the last reset
+    uint64_t peak = 0;    // high-water mark of in_use + cached since the last reset
+    uint64_t cached = 0;  // bytes of freed blocks kept for reuse
+    std::multimap<size_t, void*> cache;    // freed blocks by size
+    std::map<void*, size_t> blocks;        // live blocks and their sizes
     std::string err;
     // reusable scratch buffers keyed by purpose
     std::map<std::string, std::pair<void*, size_t>> scratch;
@@ -101,7 +104,8 @@ void* dalloc(vt_ctx* ctx, size_t bytes);
 void dfree(vt_ctx* ctx, void* p, size_t bytes);
 // Grow-only named scratch buffer.
 void* scratch(vt_ctx* ctx, const std::string& key, size_t bytes);
-void release_scratch(vt_ctx* ctx);
+void release_scratch(vt_ctx* ctx);  // also returns cached blocks to the driver
+void trim_cache(vt_ctx* ctx);
 
 template <class T>
 struct DBuf {

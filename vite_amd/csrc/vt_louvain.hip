@@ -317,6 +317,9 @@ __device__ __forceinline__ void move_vertex(const MoveArgs& a, const Table<G, HB
 #ifndef VT_NCH_G16
 #define VT_NCH_G16 3
 #endif
+#ifndef VT_CAP_G8
+#define VT_CAP_G8 32
+#endif
 #ifndef VT_NCH_G32
 #define VT_NCH_G32 2
 #endif
@@ -579,7 +582,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                                                   const int32_t* __restrict__ list, int64_t n, int64_t idx,
                                                   int64_t stride, int lane, int gbase, unsigned* inactive_local)
 {
-    static_assert(CAP == 4 * G && CAP <= 256, "occ holds 8-bit slot indexes");
+    static_assert(CAP >= 4 * G && CAP <= 256, "occ holds 8-bit slot indexes");
     static_assert(NCH * G <= 3 * CAP / 4, "at most 3/4 of the table filled by distinct communities");
     if (idx >= n) return;
     VHeader<BATCH> h = load_header<BATCH>(a, list[idx], true);
@@ -802,10 +805,10 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
 // that its larger tables do not set the LDS footprint, and so the occupancy, of this one.
 constexpr int kFusedBuckets = 4;
 constexpr int kFusedBlock = 256;
-constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256, kNch[0]>::bytes, GroupLds<8, 32, 256, kNch[1]>::bytes,
+constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256, kNch[0]>::bytes, GroupLds<8, VT_CAP_G8, 256, kNch[1]>::bytes,
                                        GroupLds<16, 64, 256, kNch[2]>::bytes, GroupLds<32, 128, 256, kNch[3]>::bytes});
 constexpr size_t kMidLds = GroupLds<64, 256, 256, kNch[4]>::bytes;
-static_assert(kFusedLds <= 20480, "eight 256-lane blocks per CU");
+static_assert(kFusedLds <= 20480 || VT_CAP_G8 > 32, "eight 256-lane blocks per CU");
 
 struct SweepPlan {
     int64_t bbeg[kFusedBuckets + 1];  // block ranges
@@ -829,7 +832,7 @@ k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
     const int32_t* list = lists + plan.off[b];
     switch (b) {
         case 0: run_bucket<4, 16, 256, true, BATCH, kNch[0]>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
-        case 1: run_bucket<8, 32, 256, true, BATCH, kNch[1]>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+        case 1: run_bucket<8, VT_CAP_G8, 256, true, BATCH, kNch[1]>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
         case 2: run_bucket<16, 64, 256, true, BATCH, kNch[2]>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
         default: run_bucket<32, 128, 256, true, BATCH, kNch[3]>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
     }
