@@ -17,6 +17,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "vt_internal.hpp"
@@ -192,22 +193,43 @@ __global__ void __launch_bounds__(64 * kFoldWaves) k_internal_fold(
     __shared__ double st[kFoldWaves][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int64_t c = (int64_t)blockIdx.x * kFoldWaves + w; c < nc; c += (int64_t)gridDim.x * kFoldWaves) {
-        const int64_t m0 = cstart[c], m1 = cstart[c + 1];
+        const int64_t m0 = cstart[c], m1 = cstart[c + 1], nm = m1 - m0;
         const int64_t s0 = moff[m0], s1 = moff[m1];
+        // up to 64 members: lane i holds member i's start in the concatenated row and its
+        // row start in the CSR, and lanes find their member with shuffles
+        const bool small = nm <= 64;
+        int64_t my_s = INT64_MAX, my_row = 0;
+        if (small && lane < nm) {
+            my_s = moff[m0 + lane];
+            my_row = offs[mem[m0 + lane]];
+        }
         double acc = -0.0;
         bool any = false;
         for (int64_t base = s0; base < s1; base += 64) {
             const int64_t p = base + lane;
             double x = -0.0;
             bool in = false;
-            if (p < s1) {
+            int64_t k = 0;
+            if (small) {
+                int lo = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int cand = lo + step;
+                    const int64_t cs = __shfl(my_s, cand < 64 ? cand : 63);
+                    if (cand < nm && cs <= p) lo = cand;
+                }
+                k = __shfl(my_row, lo) + (p - __shfl(my_s, lo));
+            } else if (p < s1) {
                 int64_t lo = m0, hi = m1 - 1;  // the member whose row holds position p
                 while (lo < hi) {
                     const int64_t mid = (lo + hi + 1) >> 1;
                     if (moff[mid] <= p) lo = mid;
                     else hi = mid - 1;
                 }
-                const DEdge e = edges[offs[mem[lo]] + (p - moff[lo])];
+                k = offs[mem[lo]] + (p - moff[lo]);
+            }
+            if (p < s1) {
+                const DEdge e = edges[k];
                 in = newid[e.tail] == (uint32_t)c;
                 if (in) x = e.weight;
             }
