@@ -263,3 +263,21 @@ def test_autotune_keeps_results(ctx):
         assert np.array_equal(cv, cv_o)
     finally:
         ctx.set_tuning(**default)
+
+
+@pytest.mark.parametrize("name", ["rgg", "ba", "dense", "multi", "grid"])
+def test_lane_per_vertex_kernel(ctx, name, monkeypatch):
+    """The lane-per-vertex kernel (selected for rows up to VITE_AMD_LPV_MAXDEG entries) gives
+    the same sweep and phase as the oracle."""
+    monkeypatch.setenv("VITE_AMD_LPV_MAXDEG", "64")
+    g, dg = _upload(ctx, name)
+    curr, csize, cdeg, vdeg, c = graphs.random_state(g, 3)
+    t_o, cw_o, ds_o, dd_o = po.sweep(g, curr, csize, cdeg, vdeg, c)
+    t, cw, ds, dd = dg.sweep(curr, csize, cdeg, vdeg, c)
+    assert np.array_equal(t, t_o)
+    assert np.array_equal(cw.view(np.uint64), cw_o.view(np.uint64))
+    assert np.array_equal(dd.view(np.uint64), dd_o.view(np.uint64))
+    q_o, cv_o, it_o = po.louvain_phase(g, et_type=1)
+    q, cv, it = dg.louvain_phase(sequential=True, et_type=1)
+    assert (q, it) == (q_o, it_o)
+    assert np.array_equal(cv, cv_o)
