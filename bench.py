@@ -123,6 +123,12 @@ def batched_form(ctx, args):
     gen_s = time.perf_counter() - t0
     b = vite_amd.Batch(ctx, graphs)
     log(f"batch: {count} graphs, {b.total_nv} vertices, {b.total_ne} directed entries (generated in {gen_s:.1f}s)")
+    saved = ctx.tuning
+    tuning = saved
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = b.autotune(prep=1, reps=2)
+        log(f"batch autotune: {tuning} in {time.perf_counter() - t0:.2f}s")
     kw = dict(threshold_cycling=True)
     t0 = time.perf_counter()
     res, st = b.run(**kw)
@@ -140,7 +146,8 @@ def batched_form(ctx, args):
     b.free()
     for g in graphs:
         g.free()
-    return {"graphs": count, "graphs_per_s": count / batch_s, "seconds": batch_s,
+    ctx.set_tuning(**saved)
+    return {"graphs": count, "graphs_per_s": count / batch_s, "seconds": batch_s, "tuning": tuning,
             "one_per_launch_graphs_per_s": plain_gps, "one_per_launch_sample": len(sample),
             "speedup": (count / batch_s) / plain_gps, "identical_per_graph_results": same,
             "phases": st["phases"], "rounds": st["rounds"], "total_nv": b.total_nv, "total_ne": b.total_ne,
@@ -159,6 +166,12 @@ def skewed_form(ctx, args):
     gen_s = time.perf_counter() - t0
     deg = np.diff(dg.offsets())
     log(f"skewed: BA n={dg.nv} m=8, {dg.ne} entries, max degree {int(deg.max())} (generated in {gen_s:.2f}s)")
+    saved = ctx.tuning
+    tuning = saved
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = ctx.autotune(dg, prep=args.prep, reps=3)
+        log(f"skewed autotune: {tuning} in {time.perf_counter() - t0:.2f}s")
     st = dg.phase_begin(lower=-1.0, thresh=1e-3)
     for _ in range(args.prep):
         st.step()
@@ -181,12 +194,13 @@ def skewed_form(ctx, args):
     r = dg.run(coloring=True, max_colors=8, et_type=2, et_delta=0.25)
     run_s = time.perf_counter() - t0
     dg.free()
+    ctx.set_tuning(**saved)
     log(f"skewed: sweep {1e3 * wall:.3f} ms ({dg.ne / wall / 1e9:.2f} G edges/s) {kernels}; "
         f"-c 8 -t 2 -a 0.25 run {run_s:.2f}s Q={r.final_q:.9f}")
     return {"n": int(len(deg)), "directed_entries": int(deg.sum()), "max_degree": int(deg.max()),
             "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall, "kernel_avg_ms": kernels,
             "run_seconds": run_s, "final_q": r.final_q, "levels": len(r.level_q), "colors": r.num_colors,
-            "flags": "-c 8 -t 2 -a 0.25"}
+            "flags": "-c 8 -t 2 -a 0.25", "tuning": tuning}
 
 
 def serving_form(args):
@@ -224,8 +238,9 @@ def serving_form(args):
         return out, {"p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
                      "graphs_per_s": count / span, "makespan_s": span}
 
-    # capacity of the sequential solver: the requests back to back
+    # capacities: the requests all at once, one slot (the sequential solver) and all slots
     _, closed = replay(1, [0.0] * count)
+    _, closed_mix = replay(args.serve_slots, [0.0] * count)
     lam = args.serve_load * closed["graphs_per_s"]
     arrivals = np.cumsum(rng.exponential(1.0 / lam, size=count))
     seq_out, seq = replay(1, arrivals)
@@ -233,12 +248,15 @@ def serving_form(args):
     same = all(a.call_q == b.call_q and a.call_iters == b.call_iters and a.final_q == b.final_q
                for (a, _), (b, _) in zip(seq_out, mix_out))
     ok = all(i["status"] == 0 for _, i in mix_out)
+    log(f"serving: capacity {closed['graphs_per_s']:.1f} graphs/s sequential, {closed_mix['graphs_per_s']:.1f} with "
+        f"{args.serve_slots} slots")
     log(f"serving: offered {lam:.1f} req/s; sequential p50 {seq['p50_ms']:.1f} ms p99 {seq['p99_ms']:.1f} ms "
         f"{seq['graphs_per_s']:.1f} graphs/s; {args.serve_slots} slots p50 {mix['p50_ms']:.1f} ms "
         f"p99 {mix['p99_ms']:.1f} ms {mix['graphs_per_s']:.1f} graphs/s; identical={same}")
     return {"requests": count, "offered_per_s": lam, "slots": args.serve_slots, "budget_bytes": budget,
             "p50_ms": mix["p50_ms"], "p99_ms": mix["p99_ms"], "graphs_per_s": mix["graphs_per_s"],
             "sequential": seq, "sequential_capacity_per_s": closed["graphs_per_s"],
+            "capacity_per_s": closed_mix["graphs_per_s"],
             "identical_alone_vs_mix": same, "all_ok": ok,
             "workload": "Poisson arrivals; RGG or BA (m=8) 50/50, n log-uniform in [2^10, 2^20]; threshold cycling"}
 

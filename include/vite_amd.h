@@ -235,6 +235,9 @@ int vt_batch_create(vt_ctx* ctx, int32_t n, vt_graph* const* graphs, vt_batch** 
 int vt_batch_info(const vt_batch* b, int32_t* n, int64_t* total_nv, int64_t* total_ne);
 /* results: n entries, in the order the graphs were given.  stats may be NULL. */
 int vt_batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* p, vt_run_result* results, vt_batch_stats* stats);
+/* vt_autotune on the batch's level-0 union, its vertices swept as one graph: picks the
+ * bucket bounds and block cap for the batch's degree mix and keeps them on ctx. */
+int vt_batch_autotune(vt_ctx* ctx, vt_batch* b, int32_t prep, int32_t reps, vt_tuning* out);
 int vt_batch_free(vt_ctx* ctx, vt_batch* b);
 
 /* ---- serving form (no reference counterpart: Vite runs one job per process) ------------ */

@@ -91,6 +91,24 @@ def test_batch_order_and_repeats(ctx, mixed):
         _same(a, c)
 
 
+def test_batch_autotune_keeps_results(ctx, mixed):
+    """The batch's own autotuning changes launch shapes only: every graph still gets what it
+    gets alone under the default setting."""
+    host, dev = mixed
+    default = ctx.tuning
+    alone = [g.run(threshold_cycling=True) for g in dev]
+    b = vite_amd.Batch(ctx, dev)
+    try:
+        t = b.autotune(prep=1, reps=1)
+        assert t == ctx.tuning
+        res, _ = b.run(threshold_cycling=True)
+    finally:
+        b.free()
+        ctx.set_tuning(**default)
+    for a, r in zip(alone, res):
+        _same(a, r)
+
+
 def test_batch_rejects(ctx):
     g_empty = ctx.upload(vite_amd.CSR(np.zeros(2, np.int64), np.zeros(0, po.EDGE_DTYPE)))
     with pytest.raises(vite_amd.VtError):

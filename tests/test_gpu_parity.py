@@ -115,6 +115,22 @@ def test_rebuild(ctx, name):
 
 
 @pytest.mark.parametrize("name", graphs.ALL)
+@pytest.mark.parametrize("part", ["identity", "random", "one"])
+def test_rebuild_partitions(ctx, name, part):
+    """Partitions no phase produces: singletons (every fine row becomes a coarse row; the hub
+    has more distinct neighbours than the per-community table holds, so it takes the sort
+    path), random communities, and one community holding every vertex."""
+    g, dg = _upload(ctx, name)
+    rng = np.random.default_rng(7)
+    cv = {"identity": np.arange(g.nv), "random": rng.integers(0, max(1, g.nv // 3), g.nv),
+          "one": np.zeros(g.nv)}[part].astype(np.int64)
+    r_o = po.rebuild(g, cv)
+    r = dg.rebuild(cv).download()
+    assert np.array_equal(r.offs, r_o.offs)
+    assert np.array_equal(r.edges.view(np.uint8), r_o.edges.view(np.uint8))
+
+
+@pytest.mark.parametrize("name", graphs.ALL)
 def test_rebuild_from_device_partition(ctx, name):
     g, dg = _upload(ctx, name)
     _, cv, _ = dg.louvain_phase(sequential=True)

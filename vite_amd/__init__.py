@@ -120,6 +120,7 @@ _SIGS = {
     "vt_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "vt_batch_run": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_RunParams), C.POINTER(_RunResult),
                                C.POINTER(_BatchStats)]),
+    "vt_batch_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.POINTER(_Tuning)]),
     "vt_batch_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vt_request_reservation": (C.c_uint64, [C.POINTER(_Request)]),
     "vt_server_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
@@ -420,6 +421,12 @@ class Batch:
         st = _BatchStats()
         self.ctx.check(_lib.vt_batch_run(self.ctx.h, self.h, C.byref(rp), res, C.byref(st)))
         return [_run_result(r) for r in res], dict(phases=st.phases, rounds=st.rounds, seconds=st.seconds)
+
+    def autotune(self, prep: int = 1, reps: int = 2) -> dict:
+        """Picks the launch tuning for this batch's degree mix and keeps it on the context."""
+        t = _Tuning()
+        self.ctx.check(_lib.vt_batch_autotune(self.ctx.h, self.h, prep, reps, C.byref(t)))
+        return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket)
 
     def free(self):
         if self.h:

@@ -424,6 +424,15 @@ int vt_batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* p, vt_run_result
     });
 }
 
+int vt_batch_autotune(vt_ctx* ctx, vt_batch* b, int32_t prep, int32_t reps, vt_tuning* out)
+{
+    if (!ctx || !b) return VT_ERR_ARG;
+    // The launch shapes depend on the degree mix, which the level-0 union carries: its sweeps
+    // run as one graph (one 1/2m) touch the same rows and tables as the batched sweeps, and
+    // no setting changes a result.
+    return vt_autotune(ctx, b->u, prep, reps, out);
+}
+
 int vt_batch_free(vt_ctx* ctx, vt_batch* b)
 {
     if (!ctx) return VT_ERR_ARG;

@@ -165,6 +165,30 @@ inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)
     return (unsigned)g;
 }
 
+// Folds the staged weights of the lanes in `rest` (ascending lane = edge order) onto acc,
+// U terms per round so the LDS latency is paid once per round.  Rounds past the last term
+// read wst[-1] = -0.0, and x + (-0.0) == x for every x, so no select is needed.
+template <class mask_t, int U>
+__device__ __forceinline__ double fold_rest(double acc, mask_t rest, const double* wst)
+{
+    while (rest) {
+        int k[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            k[u] = rest ? (int)(sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)rest)
+                                                    : __builtin_ctz((unsigned)rest))
+                        : -1;
+            rest = (mask_t)(rest & (rest - 1));
+        }
+        double w[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) w[u] = wst[k[u]];
+#pragma unroll
+        for (int u = 0; u < U; u++) acc = acc + w[u];
+    }
+    return acc;
+}
+
 // ---- device primitives (vt_prims.hip) -------------------------------------------------
 // Deterministic fixed-order sums: every block reduces a fixed contiguous chunk with a
 // fixed tree; block partials are reduced by one block in index order.

@@ -394,30 +394,6 @@ struct GroupTable {
     uint8_t* tocc;
 };
 
-// Folds the staged weights of the lanes in `rest` (ascending lane = edge order) onto acc,
-// U terms per round so the LDS latency is paid once per round.  Rounds past the last term
-// read wst[-1] = -0.0, and x + (-0.0) == x for every x, so no select is needed.
-template <class mask_t, int U>
-__device__ __forceinline__ double fold_rest(double acc, mask_t rest, const double* wst)
-{
-    while (rest) {
-        int k[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            k[u] = rest ? (int)(sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)rest)
-                                                    : __builtin_ctz((unsigned)rest))
-                        : -1;
-            rest = (mask_t)(rest & (rest - 1));
-        }
-        double w[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) w[u] = wst[k[u]];
-#pragma unroll
-        for (int u = 0; u < U; u++) acc = acc + w[u];
-    }
-    return acc;
-}
-
 #ifndef VT_FOLD_UNROLL
 #define VT_FOLD_UNROLL 2
 #endif

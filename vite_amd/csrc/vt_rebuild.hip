@@ -4,14 +4,13 @@
 //     vertex index ascends (distReNumber, rebuild.cpp:121-135) -- atomicMin of the first
 //     vertex per community, a flag scan over vertices, one gather;
 //  2. coarse edge (C1, C2) weight = left fold over the vertices of C1 ascending, each row in
-//     CSR order (fill_newEdgesMap, rebuild.cpp:294-329), the first contribution as is:
-//     - internal edges (C2 == C1, most of them after a phase): one thread per community
-//       walks its members (a stable sort of vertices by new id) and folds them in place;
-//     - external edges: compacted in global edge order, radix-sorted stably by (C1, C2),
-//       and every run folded left to right by one thread;
+//     CSR order (fill_newEdgesMap, rebuild.cpp:294-329), the first contribution as is: one
+//     wave per community walks its members' rows in that order and folds the self-loop
+//     weight and, in an LDS table, every external (C1, C2) entry; communities with too many
+//     distinct neighbours for the table take a stable radix sort of their external entries
+//     by (C1, C2) whose runs are folded left to right;
 //  3. rows sorted by C2 (std::map order, rebuild.cpp:469-476), the self-loop entry placed at
-//     its column among the external runs.  Self-loops are kept.
-// Graphs with 2^31 or more edges take the older path that sorts every edge by (C1, C2).
+//     its column.  Self-loops are kept.
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -23,11 +22,6 @@
 #include "vt_internal.hpp"
 
 namespace vt {
-
-#ifndef VT_SPLIT_REBUILD
-#define VT_SPLIT_REBUILD 1
-#endif
-constexpr bool kSplitRebuild = VT_SPLIT_REBUILD;
 
 __global__ void k_first_vertex(const int64_t* __restrict__ cvect, int64_t nv, unsigned long long* __restrict__ first,
                                unsigned long long* __restrict__ bad)
@@ -64,44 +58,6 @@ __global__ void k_total(const int64_t* __restrict__ flag, const int64_t* __restr
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = n ? scan[n - 1] + flag[n - 1] : 0;
 }
 
-// One key per fine edge: (C1 << bits) | C2, value = weight.  A block takes 256 consecutive
-// rows, caches their starts and new ids in LDS, and walks their (contiguous) edges with
-// consecutive threads on consecutive edges; each thread finds its edge's row by binary
-// search over the cached starts.
-constexpr int kRowsPerBlock = 256;
-
-__global__ void __launch_bounds__(kRowsPerBlock) k_edge_keys(const int64_t* __restrict__ offs,
-                                                             const DEdge* __restrict__ edges, int64_t nv,
-                                                             const uint32_t* __restrict__ newid, int bits,
-                                                             unsigned long long* __restrict__ keys,
-                                                             double* __restrict__ vals)
-{
-    __shared__ int64_t so[kRowsPerBlock + 1];
-    __shared__ uint32_t sid[kRowsPerBlock];
-    for (int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock; r0 < nv; r0 += (int64_t)gridDim.x * kRowsPerBlock) {
-        const int nr = (int)(nv - r0 < kRowsPerBlock ? nv - r0 : kRowsPerBlock);
-        if ((int)threadIdx.x < nr) {
-            so[threadIdx.x] = offs[r0 + threadIdx.x];
-            sid[threadIdx.x] = newid[r0 + threadIdx.x];
-        }
-        if (threadIdx.x == 0) so[nr] = offs[r0 + nr];
-        __syncthreads();
-        const int64_t e1 = so[nr];
-        for (int64_t k = so[0] + threadIdx.x; k < e1; k += kRowsPerBlock) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (so[mid] <= k) lo = mid;
-                else hi = mid - 1;
-            }
-            const DEdge e = edges[k];
-            keys[k] = ((unsigned long long)sid[lo] << bits) | newid[e.tail];
-            vals[k] = e.weight;
-        }
-        __syncthreads();
-    }
-}
-
 __global__ void k_run_heads(const unsigned long long* __restrict__ keys, int64_t n, int64_t* __restrict__ head)
 {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
@@ -117,40 +73,20 @@ __global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __
     if (blockIdx.x == 0 && threadIdx.x == 0) starts[nu] = n;
 }
 
-// Run r: left fold of its weights in sorted (= fine edge) order, the first contribution as
-// is; the bounds are known up front, so the loads run ahead of the dependent adds.
-__global__ void k_run_fold(const unsigned long long* __restrict__ keys, const double* __restrict__ vals,
-                           const int64_t* __restrict__ starts, int64_t nu, int bits, DEdge* __restrict__ out,
-                           unsigned long long* __restrict__ rowcount)
-{
-    const unsigned long long lomask = (1ull << bits) - 1;
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nu; r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = starts[r], e = starts[r + 1];
-        const unsigned long long key = keys[b];
-        double acc = vals[b];
-        int64_t k = b + 1;
-        for (; k + 4 <= e; k += 4) {
-            const double x0 = vals[k], x1 = vals[k + 1], x2 = vals[k + 2], x3 = vals[k + 3];
-            acc = acc + x0;
-            acc = acc + x1;
-            acc = acc + x2;
-            acc = acc + x3;
-        }
-        for (; k < e; k++) acc = acc + vals[k];
-        DEdge o;
-        o.tail = (int64_t)(key & lomask);
-        o.weight = acc;
-        out[r] = o;
-        atomicAdd(&rowcount[key >> bits], 1ull);
-    }
-}
-
-// ---- internal / external split -------------------------------------------------------------
-// Most fine edges end inside their own community.  Their coarse self-loop weight is the
-// left fold of the community's internal edge weights over its members in vertex order,
-// each row in CSR order: one thread per community walks its members directly, with no sort.
-// Only the external edges (C1 != C2) go through the stable (C1, C2) radix sort.  The
-// self-loop entry is then placed at its sorted column in the coarse row.
+// ---- per-community fold --------------------------------------------------------------------
+// The members of each community (a stable sort of vertices by new id) in vertex order, each
+// member's row in CSR order, form one concatenated row per community: exactly the order in
+// which rebuild.cpp:303-327 adds the community's fine edges into its coarse row.  One wave
+// walks a community's concatenated row once, 64 entries at a time with coalesced loads, and
+// folds every coarse entry (C2 = new id of the tail, the community itself included) into a
+// per-wave LDS table as the local move folds its counters: lanes insert their key by CAS,
+// mark their lane in the slot's mask, and the lowest lane of each key adds the chunk's
+// weights for it in lane (= row) order onto the slot's value, which starts at -0.0
+// (-0.0 + w == w, so each entry is the left fold with the first weight as is).  The coarse
+// row, the table's keys in ascending order, goes to a staging area at the community's
+// offset in the concatenated rows, which is at least as long.  A community with more than
+// kCfMaxKeys distinct keys takes a stable radix sort of its entries by (C1, C2) instead,
+// whose runs are folded left to right.
 
 __global__ void k_count_members(const uint32_t* __restrict__ newid, int64_t nv, int64_t* __restrict__ cnt)
 {
@@ -171,13 +107,6 @@ __global__ void k_member_degrees(const uint32_t* __restrict__ mem, int64_t nv, c
         mdeg[m] = m < nv ? offs[mem[m] + 1] - offs[mem[m]] : 0;
 }
 
-// One wave per community walks its members' rows as one concatenated row (moff: start of
-// each member's row in it), 64 entries at a time with coalesced loads.  Each lane stages
-// its weight, or -0.0 when the edge leaves the community, and lane 0 adds the staged values
-// in order onto a sum that starts at -0.0.  -0.0 + w == w and x + (-0.0) == x for every
-// double, so the result is the left fold of the internal weights with the first one as is.
-constexpr int kFoldWaves = 4;
-
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -185,111 +114,178 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ void __launch_bounds__(64 * kFoldWaves) k_internal_fold(
-    int64_t nc, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
-    const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
-    double* __restrict__ self_w, unsigned char* __restrict__ has_self)
-{
-    __shared__ double st[kFoldWaves][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int64_t c = (int64_t)blockIdx.x * kFoldWaves + w; c < nc; c += (int64_t)gridDim.x * kFoldWaves) {
-        const int64_t m0 = cstart[c], m1 = cstart[c + 1], nm = m1 - m0;
-        const int64_t s0 = moff[m0], s1 = moff[m1];
-        // up to 64 members: lane i holds member i's start in the concatenated row and its
-        // row start in the CSR, and lanes find their member with shuffles
-        const bool small = nm <= 64;
-        int64_t my_s = INT64_MAX, my_row = 0;
-        if (small && lane < nm) {
+// Positions of community c's concatenated row -> CSR indices.  Up to 64 members, lane i
+// holds member i's start in the concatenated row and its row start in the CSR, and lanes
+// find their member with shuffles; larger communities search moff.
+struct MemberRows {
+    int64_t m0, m1, s0, s1, my_s = INT64_MAX, my_row = 0;
+    bool small;
+
+    __device__ MemberRows(const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem,
+                          const int64_t* __restrict__ moff, const int64_t* __restrict__ offs, int64_t c, int lane)
+    {
+        m0 = cstart[c];
+        m1 = cstart[c + 1];
+        s0 = moff[m0];
+        s1 = moff[m1];
+        small = m1 - m0 <= 64;
+        if (small && lane < m1 - m0) {
             my_s = moff[m0 + lane];
             my_row = offs[mem[m0 + lane]];
         }
-        double acc = -0.0;
-        bool any = false;
-        for (int64_t base = s0; base < s1; base += 64) {
-            const int64_t p = base + lane;
-            double x = -0.0;
-            bool in = false;
-            int64_t k = 0;
-            if (small) {
-                int lo = 0;
+    }
+
+    // CSR index of position p (p < s1); every lane of the wave calls it.
+    __device__ int64_t csr_index(int64_t p, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
+                                 const int64_t* __restrict__ offs) const
+    {
+        if (small) {
+            const int nm = (int)(m1 - m0);
+            int lo = 0;
 #pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    const int cand = lo + step;
-                    const int64_t cs = __shfl(my_s, cand < 64 ? cand : 63);
-                    if (cand < nm && cs <= p) lo = cand;
-                }
-                k = __shfl(my_row, lo) + (p - __shfl(my_s, lo));
-            } else if (p < s1) {
-                int64_t lo = m0, hi = m1 - 1;  // the member whose row holds position p
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi + 1) >> 1;
-                    if (moff[mid] <= p) lo = mid;
-                    else hi = mid - 1;
-                }
-                k = offs[mem[lo]] + (p - moff[lo]);
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = lo + step;
+                const int64_t cs = __shfl(my_s, cand < 64 ? cand : 63);
+                if (cand < nm && cs <= p) lo = cand;
             }
-            if (p < s1) {
+            return __shfl(my_row, lo) + (p - __shfl(my_s, lo));
+        }
+        if (p >= s1) return 0;
+        int64_t lo = m0, hi = m1 - 1;  // the member whose row holds position p
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (moff[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        return offs[mem[lo]] + (p - moff[lo]);
+    }
+};
+
+constexpr int kCfWaves = 4;
+constexpr int kCfLog = 8;
+constexpr int kCfCap = 1 << kCfLog;       // table slots per wave
+constexpr int kCfMaxKeys = 128;           // distinct keys a community may hold in the table
+constexpr int kCfList = kCfMaxKeys + 64;  // a chunk adds at most 64 keys before the check
+constexpr uint32_t kNoKey = 0xffffffffu;
+
+// Per community c: its coarse row staged at rkey/rval[s0, s0 + rowlen[c]), with s0 its
+// offset in the concatenated rows; or, with more distinct keys than the table holds,
+// rowlen[c] = 0 and xcnt[c] = its entry count, for the sort path.
+__global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
+    int64_t nc, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
+    const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
+    uint32_t* __restrict__ rkey, double* __restrict__ rval, int64_t* __restrict__ rowlen, int64_t* __restrict__ xcnt)
+{
+    __shared__ uint32_t tk[kCfWaves][kCfCap];
+    __shared__ double tv[kCfWaves][kCfCap];
+    __shared__ unsigned long long tm[kCfWaves][kCfCap];
+    __shared__ uint32_t tl[kCfWaves][kCfList];  // keys in insertion order
+    __shared__ uint16_t ts[kCfWaves][kCfList];  // and their slots
+    __shared__ double wst[kCfWaves][65];        // [0] = -0.0, then the chunk's weights
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1;
+    for (int i = lane; i < kCfCap; i += 64) {
+        tk[w][i] = kNoKey;
+        tv[w][i] = -0.0;
+        tm[w][i] = 0;
+    }
+    if (lane == 0) wst[w][0] = -0.0;
+    wave_lds_sync();
+    for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
+        const MemberRows mr(cstart, mem, moff, offs, c, lane);
+        int nd = 0;  // distinct keys so far (the table never holds more than kCfList)
+        for (int64_t base = mr.s0; base < mr.s1 && nd <= kCfMaxKeys; base += 64) {
+            const int64_t p = base + lane;
+            const int64_t k = mr.csr_index(p, mem, moff, offs);
+            const bool valid = p < mr.s1;
+            uint32_t c2 = 0;
+            double wt = -0.0;
+            if (valid) {
                 const DEdge e = edges[k];
-                in = newid[e.tail] == (uint32_t)c;
-                if (in) x = e.weight;
+                c2 = newid[e.tail];
+                wt = e.weight;
             }
-            any = any || (__ballot(in) != 0);
-            st[w][lane] = x;
+            int slot = 0;
+            bool fresh = false;
+            if (valid) {
+                uint32_t h = (c2 * 0x9E3779B1u) >> (32 - kCfLog);
+                for (;;) {  // ends: at most kCfList < kCfCap keys are ever present
+                    const uint32_t old = atomicCAS(&tk[w][h], kNoKey, c2);
+                    if (old == kNoKey || old == c2) {
+                        fresh = old == kNoKey;
+                        break;
+                    }
+                    h = (h + 1) & (kCfCap - 1);
+                }
+                slot = (int)h;
+            }
+            const unsigned long long fm = __ballot(fresh);
+            if (fresh) {
+                const int idx = nd + __popcll(fm & below);
+                tl[w][idx] = c2;
+                ts[w][idx] = (uint16_t)slot;
+            }
+            nd += __popcll(fm);
+            wst[w][1 + lane] = wt;
+            if (valid) atomicOr(&tm[w][slot], 1ull << lane);
             wave_lds_sync();
+            const unsigned long long m = valid ? tm[w][slot] : 0ull;
+            wave_lds_sync();
+            if (valid && (m & below) == 0) {
+                tv[w][slot] = fold_rest<unsigned long long, 2>(tv[w][slot], m, &wst[w][1]);
+                tm[w][slot] = 0;
+            }
+            wave_lds_sync();
+        }
+        if (nd <= kCfMaxKeys) {
+            // each key's column is its rank among the (distinct) keys
+            for (int i = lane; i < nd; i += 64) {
+                const uint32_t ki = tl[w][i];
+                int rank = 0;
+                for (int j = 0; j < nd; j++) rank += tl[w][j] < ki ? 1 : 0;
+                rkey[mr.s0 + rank] = ki;
+                rval[mr.s0 + rank] = tv[w][ts[w][i]];
+            }
             if (lane == 0) {
-                const int n = (int)(s1 - base < 64 ? s1 - base : 64);
-                for (int i = 0; i < n; i++) acc = acc + st[w][i];
+                rowlen[c] = nd;
+                xcnt[c] = 0;
             }
-            wave_lds_sync();
+        } else if (lane == 0) {
+            rowlen[c] = 0;
+            xcnt[c] = mr.s1 - mr.s0;
         }
-        if (lane == 0) {
-            self_w[c] = acc;
-            has_self[c] = any ? 1 : 0;
+        wave_lds_sync();
+        for (int i = lane; i < nd; i += 64) {
+            const int sl = ts[w][i];
+            tk[w][sl] = kNoKey;
+            tv[w][sl] = -0.0;
         }
+        wave_lds_sync();
     }
 }
 
-// Edge-parallel over blocks of 256 rows (row starts and new ids cached in LDS): pass 0 flags
-// the external edges, pass 1 writes their (C1 << bits | C2, weight) at their compacted
-// positions, keeping global edge order.
-template <int PASS>
-__global__ void __launch_bounds__(kRowsPerBlock) k_ext_edges(const int64_t* __restrict__ offs,
-                                                             const DEdge* __restrict__ edges, int64_t nv,
-                                                             const uint32_t* __restrict__ newid, int bits,
-                                                             int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
-                                                             unsigned long long* __restrict__ keys,
-                                                             double* __restrict__ vals)
+// Every entry of the communities that overflowed the table, as ((C1 << bits) | C2, weight),
+// at xoff[C1] + its position in C1's concatenated row.
+__global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit(
+    int64_t nc, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
+    const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
+    const int64_t* __restrict__ xcnt, const int64_t* __restrict__ xoff, int bits, unsigned long long* __restrict__ keys,
+    double* __restrict__ vals)
 {
-    __shared__ int64_t so[kRowsPerBlock + 1];
-    __shared__ uint32_t sid[kRowsPerBlock];
-    for (int64_t r0 = (int64_t)blockIdx.x * kRowsPerBlock; r0 < nv; r0 += (int64_t)gridDim.x * kRowsPerBlock) {
-        const int nr = (int)(nv - r0 < kRowsPerBlock ? nv - r0 : kRowsPerBlock);
-        if ((int)threadIdx.x < nr) {
-            so[threadIdx.x] = offs[r0 + threadIdx.x];
-            sid[threadIdx.x] = newid[r0 + threadIdx.x];
-        }
-        if (threadIdx.x == 0) so[nr] = offs[r0 + nr];
-        __syncthreads();
-        const int64_t e1 = so[nr];
-        for (int64_t k = so[0] + threadIdx.x; k < e1; k += kRowsPerBlock) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (so[mid] <= k) lo = mid;
-                else hi = mid - 1;
-            }
-            const DEdge e = edges[k];
-            const uint32_t c2 = newid[e.tail];
-            if constexpr (PASS == 0) {
-                flag[k] = c2 != sid[lo] ? 1 : 0;
-            } else {
-                if (c2 != sid[lo]) {
-                    keys[pos[k]] = ((unsigned long long)sid[lo] << bits) | c2;
-                    vals[pos[k]] = e.weight;
-                }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
+        if (xcnt[c] == 0) continue;
+        const MemberRows mr(cstart, mem, moff, offs, c, lane);
+        const int64_t o = xoff[c] - mr.s0;
+        for (int64_t base = mr.s0; base < mr.s1; base += 64) {
+            const int64_t p = base + lane;
+            const int64_t k = mr.csr_index(p, mem, moff, offs);
+            if (p < mr.s1) {
+                const DEdge e = edges[k];
+                keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
+                vals[o + p] = e.weight;
             }
         }
-        __syncthreads();
     }
 }
 
@@ -328,181 +324,167 @@ __device__ __forceinline__ int64_t lower_bound_u32(const uint32_t* __restrict__ 
     return lo;
 }
 
-// Per community: its first external run, how many it has, and how many sort before the
-// self-loop column (runs are sorted by C1, then C2) -- binary searches, no atomics.
-__global__ void k_comm_runs(int64_t nc, const uint32_t* __restrict__ rc1, const uint32_t* __restrict__ rc2, int64_t nu,
-                            int64_t* __restrict__ ext_first, int64_t* __restrict__ ext_cnt, int64_t* __restrict__ lt_cnt)
+// An overflowing community's first run and run count (= its coarse row length); runs are
+// sorted by C1, then C2 -- binary searches, no atomics.
+__global__ void k_ovf_rows(int64_t nc, const int64_t* __restrict__ xcnt, const uint32_t* __restrict__ rc1, int64_t nu,
+                           int64_t* __restrict__ run_first, int64_t* __restrict__ rowlen)
 {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+        if (xcnt[c] == 0) continue;
         const int64_t f = lower_bound_u32(rc1, 0, nu, (uint32_t)c);
-        const int64_t l = lower_bound_u32(rc1, f, nu, (uint32_t)c + 1);
-        ext_first[c] = f;
-        ext_cnt[c] = l - f;
-        lt_cnt[c] = lower_bound_u32(rc2, f, l, (uint32_t)c) - f;
+        run_first[c] = f;
+        rowlen[c] = lower_bound_u32(rc1, f, nu, (uint32_t)c + 1) - f;
     }
 }
 
-__global__ void k_row_counts(int64_t nc, const int64_t* __restrict__ ext_cnt, const unsigned char* __restrict__ has_self,
-                             int64_t* __restrict__ rc)
-{
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c <= nc; c += (int64_t)gridDim.x * blockDim.x)
-        rc[c] = c < nc ? ext_cnt[c] + has_self[c] : 0;
-}
-
-// Run r of community C1 (its runs are contiguous, in C2 order) goes to its rank among them,
-// one further when the self-loop entry sorts before it; the self-loop entry goes after the
-// runs with C2 < C1.
-__global__ void k_place(const uint32_t* __restrict__ rc1, const uint32_t* __restrict__ rc2, const double* __restrict__ rw,
-                        int64_t nu, const int64_t* __restrict__ out_offs, const int64_t* __restrict__ ext_first,
-                        const unsigned char* __restrict__ has_self, DEdge* __restrict__ out)
+// Run r of community C1 goes to its rank among C1's runs in C1's staged row.
+__global__ void k_ovf_place(const uint32_t* __restrict__ rc1, const uint32_t* __restrict__ rc2,
+                            const double* __restrict__ rw, int64_t nu, const int64_t* __restrict__ cstart,
+                            const int64_t* __restrict__ moff, const int64_t* __restrict__ run_first,
+                            uint32_t* __restrict__ rkey, double* __restrict__ rval)
 {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nu; r += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t c1 = rc1[r], c2 = rc2[r];
-        const int64_t p = out_offs[c1] + (r - ext_first[c1]) + ((has_self[c1] && c2 > c1) ? 1 : 0);
-        DEdge o;
-        o.tail = (int64_t)c2;
-        o.weight = rw[r];
-        out[p] = o;
+        const uint32_t c1 = rc1[r];
+        const int64_t q = moff[cstart[c1]] + (r - run_first[c1]);
+        rkey[q] = rc2[r];
+        rval[q] = rw[r];
     }
 }
 
-__global__ void k_place_self(int64_t nc, const int64_t* __restrict__ out_offs, const int64_t* __restrict__ lt_cnt,
-                             const unsigned char* __restrict__ has_self, const double* __restrict__ self_w,
-                             DEdge* __restrict__ out)
+// Staged rows -> the coarse CSR, one wave per community.
+__global__ void __launch_bounds__(64 * kCfWaves) k_row_copy(int64_t nc, const int64_t* __restrict__ cstart,
+                                                           const int64_t* __restrict__ moff,
+                                                           const int64_t* __restrict__ rowlen,
+                                                           const int64_t* __restrict__ out_offs,
+                                                           const uint32_t* __restrict__ rkey,
+                                                           const double* __restrict__ rval, DEdge* __restrict__ out)
 {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-        if (!has_self[c]) continue;
-        DEdge o;
-        o.tail = c;
-        o.weight = self_w[c];
-        out[out_offs[c] + lt_cnt[c]] = o;
-    }
-}
-
-__global__ void k_total_i32(const int32_t* __restrict__ flag, const int32_t* __restrict__ scan, int64_t n, int64_t* out)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = n ? (int64_t)scan[n - 1] + flag[n - 1] : 0;
-}
-
-// The rebuild with the internal / external split (needs ne < 2^31 for 32-bit positions).
-static vt_graph* rebuild_split(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid, int64_t nc)
-{
-    const int64_t nv = g->nv, ne = g->ne;
-    const int bits = (int)bits_for(nc > 1 ? nc - 1 : 1);
-    const size_t c1n = (size_t)nc + 1, n1 = (size_t)std::max<int64_t>(nv, 1), e1 = (size_t)std::max<int64_t>(ne, 1);
-    DBuf<int64_t> tot(ctx, 2);
-
-    // members of each community, ascending vertex order (stable sort by new id)
-    DBuf<int64_t> ccnt(ctx, c1n), cstart(ctx, c1n);
-    dzero(ctx, ccnt.p, 8 * c1n);
-    VT_LAUNCH(ctx, "rb_members", k_count_members, grid_for(nv, 256, 8192), 256, 0, newid, nv, ccnt.p);
-    exclusive_scan_i64(ctx, ccnt.p, cstart.p, nc + 1);
-    ccnt.reset();
-    DBuf<uint32_t> mk(ctx, n1), mv(ctx, n1), mk2(ctx, n1), mv2(ctx, n1);
-    d2d(ctx, mk.p, newid, 4 * (size_t)nv);
-    VT_LAUNCH(ctx, "rb_iota", k_iota_u32, grid_for(nv, 256, 8192), 256, 0, mv.p, nv);
-    bool malt = false;
-    sort_pairs_u32(ctx, mk.p, mv.p, nv, bits, mk2.p, mv2.p, &malt);
-    const uint32_t* members = malt ? mv2.p : mv.p;
-
-    // self-loop weight of every community from its internal edges
-    DBuf<double> self_w(ctx, c1n);
-    DBuf<unsigned char> has_self(ctx, c1n);
-    {
-        DBuf<int64_t> mdeg(ctx, n1 + 1), moff(ctx, n1 + 1);
-        VT_LAUNCH(ctx, "rb_mdeg", k_member_degrees, grid_for(nv + 1, 256, 8192), 256, 0, members, nv,
-                  (const int64_t*)g->offs, mdeg.p);
-        exclusive_scan_i64(ctx, mdeg.p, moff.p, nv + 1);
-        VT_LAUNCH(ctx, "rb_internal", k_internal_fold, grid_for(nc, kFoldWaves, 65536), 64 * kFoldWaves, 0, nc,
-                  (const int64_t*)cstart.p, members, (const int64_t*)moff.p, (const int64_t*)g->offs,
-                  (const DEdge*)g->edges, newid, self_w.p, has_self.p);
-    }
-    mk.reset();
-    mv.reset();
-    mk2.reset();
-    mv2.reset();
-    cstart.reset();
-
-    // external edges, compacted in global edge order, then sorted stably by (C1, C2)
-    int64_t next = 0;
-    DBuf<unsigned long long> k0, k1;
-    DBuf<double> v0, v1;
-    {
-        DBuf<int32_t> flag(ctx, e1), pos(ctx, e1);
-        if (ne) {
-            VT_LAUNCH(ctx, "rb_ext_flags", k_ext_edges<0>, grid_for(nv, kRowsPerBlock, 16384), kRowsPerBlock, 0, g->offs,
-                      g->edges, nv, newid, bits, flag.p, (const int32_t*)nullptr, (unsigned long long*)nullptr,
-                      (double*)nullptr);
-            exclusive_scan_i32(ctx, flag.p, pos.p, ne);
-            VT_LAUNCH(ctx, "rb_total", k_total_i32, 1, 64, 0, (const int32_t*)flag.p, (const int32_t*)pos.p, ne, tot.p);
-            d2h(ctx, &next, tot.p, 8);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
+        const int64_t s0 = moff[cstart[c]], len = rowlen[c], o = out_offs[c];
+        for (int64_t i = lane; i < len; i += 64) {
+            DEdge e;
+            e.tail = (int64_t)rkey[s0 + i];
+            e.weight = rval[s0 + i];
+            out[o + i] = e;
         }
-        const size_t x1 = (size_t)std::max<int64_t>(next, 1);
-        k0 = DBuf<unsigned long long>(ctx, x1);
-        k1 = DBuf<unsigned long long>(ctx, x1);
-        v0 = DBuf<double>(ctx, x1);
-        v1 = DBuf<double>(ctx, x1);
-        if (next)
-            VT_LAUNCH(ctx, "rb_ext_keys", k_ext_edges<1>, grid_for(nv, kRowsPerBlock, 16384), kRowsPerBlock, 0, g->offs,
-                      g->edges, nv, newid, bits, (int32_t*)nullptr, (const int32_t*)pos.p, k0.p, v0.p);
     }
+}
+
+__global__ void k_pair_totals(const int64_t* __restrict__ a, const int64_t* __restrict__ b, int64_t* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        out[0] = *a;
+        out[1] = *b;
+    }
+}
+
+// Overflowing communities: their entries sorted stably by (C1, C2), runs folded left to
+// right, and each community's runs placed in its staged row.
+static void overflow_rows(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid, int64_t nc, int bits,
+                          const int64_t* cstart, const uint32_t* mem, const int64_t* moff, const int64_t* xcnt,
+                          const int64_t* xoff, int64_t nx, uint32_t* rkey, double* rval, int64_t* rowlen)
+{
+    const size_t x1 = (size_t)std::max<int64_t>(nx, 1), c1n = (size_t)nc + 1;
+    DBuf<unsigned long long> k0(ctx, x1), k1(ctx, x1);
+    DBuf<double> v0(ctx, x1), v1(ctx, x1);
+    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, mem, moff,
+              (const int64_t*)g->offs, (const DEdge*)g->edges, newid, xcnt, xoff, bits, k0.p, v0.p);
     const unsigned long long* ks = k0.p;
     const double* vs = v0.p;
-    if (next > 1) {
+    if (nx > 1) {
         rocprim::double_buffer<unsigned long long> kb(k0.p, k1.p);
         rocprim::double_buffer<double> vb(v0.p, v1.p);
         size_t tb = 0;
-        VT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kb, vb, (size_t)next, 0, 2 * bits, ctx->stream));
+        VT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kb, vb, (size_t)nx, 0, 2 * bits, ctx->stream));
         void* tmp = scratch(ctx, "sort_tmp", tb);
         {
             TimedScope ts(ctx, "rb_sort");
-            VT_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kb, vb, (size_t)next, 0, 2 * bits, ctx->stream));
+            VT_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kb, vb, (size_t)nx, 0, 2 * bits, ctx->stream));
         }
         ks = kb.current();
         vs = vb.current();
     }
-
-    // runs of equal (C1, C2) folded left to right
     int64_t nu = 0;
-    DBuf<int64_t> starts;
+    DBuf<int64_t> starts, tot(ctx, 2);
     {
-        const size_t x1 = (size_t)std::max<int64_t>(next, 1);
         DBuf<int64_t> head(ctx, x1), hpos(ctx, x1);
-        if (next) {
-            VT_LAUNCH(ctx, "rb_heads", k_run_heads, grid_for(next, 256, 16384), 256, 0, ks, next, head.p);
-            exclusive_scan_i64(ctx, head.p, hpos.p, next);
-            VT_LAUNCH(ctx, "rb_total", k_total, 1, 64, 0, (const int64_t*)head.p, (const int64_t*)hpos.p, next, tot.p);
-            d2h(ctx, &nu, tot.p, 8);
-        }
+        VT_LAUNCH(ctx, "rb_heads", k_run_heads, grid_for(nx, 256, 16384), 256, 0, ks, nx, head.p);
+        exclusive_scan_i64(ctx, head.p, hpos.p, nx);
+        VT_LAUNCH(ctx, "rb_total", k_total, 1, 64, 0, (const int64_t*)head.p, (const int64_t*)hpos.p, nx, tot.p);
+        d2h(ctx, &nu, tot.p, 8);
         starts = DBuf<int64_t>(ctx, (size_t)nu + 1);
-        if (next)
-            VT_LAUNCH(ctx, "rb_starts", k_run_starts, grid_for(next, 256, 16384), 256, 0, (const int64_t*)head.p,
-                      (const int64_t*)hpos.p, next, nu, starts.p);
+        VT_LAUNCH(ctx, "rb_starts", k_run_starts, grid_for(nx, 256, 16384), 256, 0, (const int64_t*)head.p,
+                  (const int64_t*)hpos.p, nx, nu, starts.p);
     }
-    DBuf<uint32_t> rc1(ctx, (size_t)std::max<int64_t>(nu, 1)), rc2(ctx, (size_t)std::max<int64_t>(nu, 1));
-    DBuf<double> rw(ctx, (size_t)std::max<int64_t>(nu, 1));
-    if (nu)
-        VT_LAUNCH(ctx, "rb_fold", k_ext_runs, grid_for(nu, 256, 16384), 256, 0, ks, vs, (const int64_t*)starts.p, nu, bits,
-                  rc1.p, rc2.p, rw.p);
+    DBuf<uint32_t> rc1(ctx, (size_t)nu), rc2(ctx, (size_t)nu);
+    DBuf<double> rw(ctx, (size_t)nu);
+    VT_LAUNCH(ctx, "rb_fold", k_ext_runs, grid_for(nu, 256, 16384), 256, 0, ks, vs, (const int64_t*)starts.p, nu, bits,
+              rc1.p, rc2.p, rw.p);
+    DBuf<int64_t> run_first(ctx, c1n);
+    VT_LAUNCH(ctx, "rb_ovf_rows", k_ovf_rows, grid_for(nc, 256, 16384), 256, 0, nc, xcnt, (const uint32_t*)rc1.p, nu,
+              run_first.p, rowlen);
+    VT_LAUNCH(ctx, "rb_ovf_place", k_ovf_place, grid_for(nu, 256, 16384), 256, 0, (const uint32_t*)rc1.p,
+              (const uint32_t*)rc2.p, (const double*)rw.p, nu, cstart, moff, (const int64_t*)run_first.p, rkey, rval);
+}
 
-    // coarse rows: the runs of C1 in C2 order with the self-loop entry at its column
-    DBuf<int64_t> ext_first(ctx, c1n), ext_cnt(ctx, c1n), lt_cnt(ctx, c1n), rc(ctx, c1n), out_offs(ctx, c1n);
-    VT_LAUNCH(ctx, "rb_comm_runs", k_comm_runs, grid_for(nc, 256, 16384), 256, 0, nc, (const uint32_t*)rc1.p,
-              (const uint32_t*)rc2.p, nu, ext_first.p, ext_cnt.p, lt_cnt.p);
-    VT_LAUNCH(ctx, "rb_rowcount", k_row_counts, grid_for(nc + 1, 256, 8192), 256, 0, nc, (const int64_t*)ext_cnt.p,
-              (const unsigned char*)has_self.p, rc.p);
-    exclusive_scan_i64(ctx, rc.p, out_offs.p, nc + 1);
-    int64_t total = 0;
-    d2h(ctx, &total, out_offs.p + nc, 8);
-    vt_graph* out = graph_new(ctx, nc, total);
+static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid, int64_t nc)
+{
+    const int64_t nv = g->nv, ne = g->ne;
+    const int bits = (int)bits_for(nc > 1 ? nc - 1 : 1);
+    const size_t c1n = (size_t)nc + 1, n1 = (size_t)std::max<int64_t>(nv, 1), e1 = (size_t)std::max<int64_t>(ne, 1);
+
+    // members of each community in ascending vertex order (stable sort by new id), and each
+    // member's start in its community's concatenated row
+    DBuf<int64_t> cstart(ctx, c1n), moff(ctx, n1 + 1);
+    DBuf<uint32_t> mem;
+    {
+        DBuf<int64_t> ccnt(ctx, c1n);
+        dzero(ctx, ccnt.p, 8 * c1n);
+        VT_LAUNCH(ctx, "rb_members", k_count_members, grid_for(nv, 256, 8192), 256, 0, newid, nv, ccnt.p);
+        exclusive_scan_i64(ctx, ccnt.p, cstart.p, nc + 1);
+        DBuf<uint32_t> mk(ctx, n1), mv(ctx, n1), mk2(ctx, n1), mv2(ctx, n1);
+        d2d(ctx, mk.p, newid, 4 * (size_t)nv);
+        VT_LAUNCH(ctx, "rb_iota", k_iota_u32, grid_for(nv, 256, 8192), 256, 0, mv.p, nv);
+        bool malt = false;
+        sort_pairs_u32(ctx, mk.p, mv.p, nv, bits, mk2.p, mv2.p, &malt);
+        mem = std::move(malt ? mv2 : mv);
+        DBuf<int64_t> mdeg(ctx, n1 + 1);
+        VT_LAUNCH(ctx, "rb_mdeg", k_member_degrees, grid_for(nv + 1, 256, 8192), 256, 0, (const uint32_t*)mem.p, nv,
+                  (const int64_t*)g->offs, mdeg.p);
+        exclusive_scan_i64(ctx, mdeg.p, moff.p, nv + 1);
+    }
+
+    // every community's coarse row, staged at its offset in the concatenated rows
+    DBuf<uint32_t> rkey(ctx, e1);
+    DBuf<double> rval(ctx, e1);
+    DBuf<int64_t> rowlen(ctx, c1n), xcnt(ctx, c1n), out_offs(ctx, c1n), xoff(ctx, c1n), tot(ctx, 2);
+    dzero(ctx, rowlen.p + nc, 8);
+    dzero(ctx, xcnt.p + nc, 8);
+    if (nc)
+        VT_LAUNCH(ctx, "rb_comm_fold", k_comm_fold, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc,
+                  (const int64_t*)cstart.p, (const uint32_t*)mem.p, (const int64_t*)moff.p, (const int64_t*)g->offs,
+                  (const DEdge*)g->edges, newid, rkey.p, rval.p, rowlen.p, xcnt.p);
+    exclusive_scan_i64(ctx, rowlen.p, out_offs.p, nc + 1);
+    exclusive_scan_i64(ctx, xcnt.p, xoff.p, nc + 1);
+    VT_LAUNCH(ctx, "rb_totals", k_pair_totals, 1, 64, 0, (const int64_t*)out_offs.p + nc, (const int64_t*)xoff.p + nc,
+              tot.p);
+    int64_t totals[2] = {0, 0};
+    d2h(ctx, totals, tot.p, 16);
+    if (totals[1]) {
+        overflow_rows(ctx, g, newid, nc, bits, cstart.p, mem.p, moff.p, xcnt.p, xoff.p, totals[1], rkey.p, rval.p,
+                      rowlen.p);
+        exclusive_scan_i64(ctx, rowlen.p, out_offs.p, nc + 1);
+        d2h(ctx, totals, out_offs.p + nc, 8);
+    }
+
+    vt_graph* out = graph_new(ctx, nc, totals[0]);
     try {
         d2d(ctx, out->offs, out_offs.p, 8 * c1n);
-        if (nu)
-            VT_LAUNCH(ctx, "rb_place", k_place, grid_for(nu, 256, 16384), 256, 0, (const uint32_t*)rc1.p,
-                      (const uint32_t*)rc2.p, (const double*)rw.p, nu, (const int64_t*)out_offs.p,
-                      (const int64_t*)ext_first.p, (const unsigned char*)has_self.p, out->edges);
-        VT_LAUNCH(ctx, "rb_place_self", k_place_self, grid_for(nc, 256, 8192), 256, 0, nc, (const int64_t*)out_offs.p,
-                  (const int64_t*)lt_cnt.p, (const unsigned char*)has_self.p, (const double*)self_w.p, out->edges);
+        if (nc)
+            VT_LAUNCH(ctx, "rb_row_copy", k_row_copy, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc,
+                      (const int64_t*)cstart.p, (const int64_t*)moff.p, (const int64_t*)rowlen.p,
+                      (const int64_t*)out_offs.p, (const uint32_t*)rkey.p, (const double*)rval.p, out->edges);
         graph_finalize(ctx, out);
     } catch (...) {
         graph_delete(ctx, out);
@@ -536,56 +518,7 @@ vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect)
     first.reset();
     flag.reset();
     rank.reset();
-    if (kSplitRebuild && ne < (int64_t(1) << 31)) return rebuild_split(ctx, g, newid.p, nc);
-
-    const int bits = (int)bits_for(nc > 1 ? nc - 1 : 1);
-    const size_t e1 = (size_t)std::max<int64_t>(ne, 1);
-    DBuf<unsigned long long> k0(ctx, e1), k1(ctx, e1);
-    DBuf<double> v0(ctx, e1), v1(ctx, e1);
-    if (nv)
-        VT_LAUNCH(ctx, "rb_keys", k_edge_keys, grid_for(nv, kRowsPerBlock, 16384), kRowsPerBlock, 0, g->offs, g->edges,
-                  nv, (const uint32_t*)newid.p, bits, k0.p, v0.p);
-    const unsigned long long* ks = k0.p;
-    const double* vs = v0.p;
-    if (ne > 1) {
-        rocprim::double_buffer<unsigned long long> kb(k0.p, k1.p);
-        rocprim::double_buffer<double> vb(v0.p, v1.p);
-        size_t tb = 0;
-        VT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kb, vb, (size_t)ne, 0, 2 * bits, ctx->stream));
-        void* tmp = scratch(ctx, "sort_tmp", tb);
-        {
-            TimedScope ts(ctx, "rb_sort");
-            VT_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kb, vb, (size_t)ne, 0, 2 * bits, ctx->stream));
-        }
-        ks = kb.current();
-        vs = vb.current();
-    }
-    DBuf<int64_t> head(ctx, e1), pos(ctx, e1);
-    int64_t nu = 0;
-    if (ne) {
-        VT_LAUNCH(ctx, "rb_heads", k_run_heads, grid_for(ne, 256, 16384), 256, 0, ks, ne, head.p);
-        exclusive_scan_i64(ctx, head.p, pos.p, ne);
-        VT_LAUNCH(ctx, "rb_total", k_total, 1, 64, 0, (const int64_t*)head.p, (const int64_t*)pos.p, ne, tot.p);
-        d2h(ctx, &nu, tot.p, 8);
-    }
-    vt_graph* out = graph_new(ctx, nc, nu);
-    try {
-        DBuf<unsigned long long> rc(ctx, (size_t)nc + 1);
-        dzero(ctx, rc.p, 8 * ((size_t)nc + 1));
-        if (ne) {
-            DBuf<int64_t> starts(ctx, (size_t)nu + 1);
-            VT_LAUNCH(ctx, "rb_starts", k_run_starts, grid_for(ne, 256, 16384), 256, 0, (const int64_t*)head.p,
-                      (const int64_t*)pos.p, ne, nu, starts.p);
-            VT_LAUNCH(ctx, "rb_fold", k_run_fold, grid_for(nu, 256, 16384), 256, 0, ks, vs, (const int64_t*)starts.p, nu,
-                      bits, out->edges, rc.p);
-        }
-        exclusive_scan_i64(ctx, (const int64_t*)rc.p, out->offs, nc + 1);
-        graph_finalize(ctx, out);
-    } catch (...) {
-        graph_delete(ctx, out);
-        throw;
-    }
-    return out;
+    return rebuild_fold(ctx, g, newid.p, nc);
 }
 
 }  // namespace vt
