@@ -48,8 +48,13 @@ __device__ __forceinline__ uint32_t slot_hash(int32_t c) { return (uint32_t)c * 
 
 // Degree buckets: (lanes per vertex G, table capacity CAP, max degree).  CAP >= 2*(deg+1)
 // keeps the table at most half full (a row has at most deg+1 distinct communities).
-constexpr int kNumBuckets = 8;
+constexpr int kNumBuckets = 10;
 constexpr int kLpvBucket = 7;      // lane per vertex (degree <= lpv_max)
+// The wide bucket (one wave per row, up to 1023 entries) is split by degree so that each row
+// gets the smallest table that keeps it at most half full: 512 slots up to degree 255, 1024
+// up to 511, else 2048.  Smaller tables leave room for more resident waves per CU.
+constexpr int kWide512Bucket = 8;
+constexpr int kWide1024Bucket = 9;
 constexpr int kLpvMaxDeg = 64;     // the largest row k_local_move_lpv is instantiated for
 constexpr int kLpvArena = 2560;
 constexpr int kHubBucket = 6;
@@ -829,12 +834,13 @@ __global__ void __launch_bounds__(256) k_local_move_mid(MoveArgs a, const int32_
 }
 
 // Bucket 5 (128 <= degree <= 1023): one wave per vertex, 2048-slot LDS table.
+template <int CAP>
 __global__ void __launch_bounds__(64) k_local_move_wide(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
 {
-    __shared__ __align__(16) char lds[LdsLayout<64, 2048, 64>::bytes];
+    __shared__ __align__(16) char lds[LdsLayout<64, CAP, 64>::bytes];
     __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
-    run_bucket<64, 2048, 64, false>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
+    run_bucket<64, CAP, 64, false>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
@@ -1025,17 +1031,18 @@ __global__ void __launch_bounds__(64) k_move_hbm(MoveArgs a, const int32_t* __re
 // reduces the argmax.  The same left folds in CSR order as distBuildLocalMapCounter
 // (louvain.cpp:2419-2466), so the same gains as the table kernels.
 
-// Keys (hub j << 32 | community of the tail) and values (edge index) for hub j's row.
+// Keys (hub j << cbits | community of the tail; communities < 2^cbits) and values (edge
+// index) for hub j's row.
 __global__ void k_hub_keys(const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
                            const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
-                           const int64_t* __restrict__ curr, unsigned long long* __restrict__ keys,
+                           const int64_t* __restrict__ curr, int cbits, unsigned long long* __restrict__ keys,
                            uint32_t* __restrict__ vals)
 {
     for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
         const int64_t q0 = hoffs[j], n = hoffs[j + 1] - q0, e0 = offs[hubs[j]];
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
             const int64_t k = e0 + i;
-            keys[q0 + i] = ((unsigned long long)j << 32) | (uint32_t)curr[edges[k].tail];
+            keys[q0 + i] = ((unsigned long long)j << cbits) | (uint32_t)curr[edges[k].tail];
             vals[q0 + i] = (uint32_t)k;
         }
     }
@@ -1097,7 +1104,7 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
                                                         const unsigned long long* __restrict__ keys,
                                                         const uint32_t* __restrict__ vals,
                                                         const int32_t* __restrict__ pos,
-                                                        const int32_t* __restrict__ starts)
+                                                        const int32_t* __restrict__ starts, int cbits)
 {
     __shared__ double s_ecc, s_self;
     __shared__ double s_bg[kHubBlock / 64];
@@ -1118,7 +1125,7 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
         const int32_t r0 = hoffs[j] < total ? pos[hoffs[j]] : nruns;
         const int32_t r1 = hoffs[j + 1] < total ? pos[hoffs[j + 1]] : nruns;
         const int64_t cc = a.curr[v];
-        const unsigned long long ckey = ((unsigned long long)j << 32) | (uint32_t)cc;
+        const unsigned long long ckey = ((unsigned long long)j << cbits) | (uint32_t)cc;
         if (threadIdx.x == 0) {
             // the own community's run, if any neighbour is in it
             int32_t lo = r0, hi = r1;
@@ -1143,7 +1150,7 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
         int32_t bc = INT_MAX;
         for (int32_t r = r0 + threadIdx.x; r < r1; r += kHubBlock) {
             const int64_t b = starts[r], e = starts[r + 1];
-            const int32_t c = (int32_t)(uint32_t)(keys[b] & 0xffffffffull);
+            const int32_t c = (int32_t)(uint32_t)(keys[b] & ((1ull << cbits) - 1));
             if ((int64_t)c == cc) continue;
             double unused = 0.0;
             const double ey = hub_run_fold(a.edges, vals, b, e, false, v, unused);
@@ -1243,7 +1250,8 @@ __global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const 
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t d = offs[i + 1] - offs[i];
-        const uint32_t b = (uint32_t)(d <= lpv_max ? kLpvBucket : degree_bucket(d, bb));
+        uint32_t b = (uint32_t)(d <= lpv_max ? kLpvBucket : degree_bucket(d, bb));
+        if (b == kHubBucket - 1) b = d <= 255 ? kWide512Bucket : d <= 511 ? kWide1024Bucket : b;
         uint32_t c = 0;
         if (colors) c = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
         keys[i] = c * kNumBuckets + b;
@@ -1460,7 +1468,8 @@ __global__ void k_batch_rotate(int64_t nv, const int32_t* __restrict__ gid, cons
 struct ColorSweep {
     SweepPlan plan{};
     int64_t fused_blocks = 0;
-    int64_t mid_off = 0, mid_n = 0, wide_off = 0, wide_n = 0, hub_off = 0, hub_n = 0, lpv_off = 0, lpv_n = 0;
+    int64_t mid_off = 0, mid_n = 0, hub_off = 0, hub_n = 0, lpv_off = 0, lpv_n = 0;
+    int64_t wide_off[3] = {0, 0, 0}, wide_n[3] = {0, 0, 0};  // 512-, 1024- and 2048-slot tables
     int32_t index = 0;  // position in Phase::sweeps
 };
 
@@ -1613,9 +1622,10 @@ struct Phase {
                 } else if (b == kFusedBuckets) {
                     cs.mid_off = off;
                     cs.mid_n = cnt;
-                } else if (b == kHubBucket - 1) {
-                    cs.wide_off = off;
-                    cs.wide_n = cnt;
+                } else if (b == kWide512Bucket || b == kWide1024Bucket || b == kHubBucket - 1) {
+                    const int w = b == kWide512Bucket ? 0 : b == kWide1024Bucket ? 1 : 2;
+                    cs.wide_off[w] = off;
+                    cs.wide_n[w] = cnt;
                 } else if (b == kLpvBucket) {
                     cs.lpv_off = off;
                     cs.lpv_n = cnt;
@@ -1734,9 +1744,12 @@ struct Phase {
             VT_LAUNCH(ctx, "local_move_mid", kern, (unsigned)((nb + kXcds - 1) / kXcds * kXcds), 256, 0, a,
                       (const int32_t*)(lists.p + cs.mid_off), cs.mid_n);
         }
-        if (cs.wide_n)
-            VT_LAUNCH(ctx, "local_move_wide", k_local_move_wide, (unsigned)std::min<int64_t>(cs.wide_n, 8192), 64, 0, a,
-                      (const int32_t*)(lists.p + cs.wide_off), cs.wide_n);
+        for (int w = 0; w < 3; w++) {
+            if (!cs.wide_n[w]) continue;
+            auto kern = w == 0 ? k_local_move_wide<512> : w == 1 ? k_local_move_wide<1024> : k_local_move_wide<2048>;
+            VT_LAUNCH(ctx, "local_move_wide", kern, (unsigned)std::min<int64_t>(cs.wide_n[w], 16384), 64, 0, a,
+                      (const int32_t*)(lists.p + cs.wide_off[w]), cs.wide_n[w]);
+        }
         if (cs.hub_n && hub_sort) {
             const int64_t total = hub_total[cs.index];
             const int32_t nh = (int32_t)cs.hub_n;
@@ -1750,10 +1763,11 @@ struct Phase {
             int32_t* head = (int32_t*)scratch(ctx, "hs_head", 4 * t1);
             int32_t* pos = (int32_t*)scratch(ctx, "hs_pos", 4 * t1);
             int32_t* starts = (int32_t*)scratch(ctx, "hs_starts", 4 * t1 + 4);
+            const int cbits = (int)bits_for(nv > 1 ? nv - 1 : 1);  // community ids < nv
             VT_LAUNCH(ctx, "hub_keys", k_hub_keys, (unsigned)std::min<int64_t>(nh, 16384), 256, 0, hubs, nh, hoffs,
-                      (const int64_t*)g->offs, (const DEdge*)g->edges, (const int64_t*)curr.p, k0, v0);
+                      (const int64_t*)g->offs, (const DEdge*)g->edges, (const int64_t*)curr.p, cbits, k0, v0);
             bool alt = false;
-            sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, 32 + (int)bits_for(nh), (uint64_t*)k1, v1, &alt);
+            sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, cbits + (int)bits_for(nh), (uint64_t*)k1, v1, &alt);
             const unsigned long long* ks = alt ? k1 : k0;
             const uint32_t* vs = alt ? v1 : v0;
             VT_LAUNCH(ctx, "hub_heads", k_hub_heads, grid_for(total, 256, 16384), 256, 0, ks, total, head);
@@ -1761,7 +1775,7 @@ struct Phase {
             VT_LAUNCH(ctx, "hub_starts", k_hub_starts, grid_for(total, 256, 16384), 256, 0, (const int32_t*)head,
                       (const int32_t*)pos, total, starts);
             VT_LAUNCH(ctx, "local_move_hub", k_hub_move, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs, nh,
-                      hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)starts);
+                      hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)starts, cbits);
         } else if (cs.hub_n) {
             const int blocks = (int)std::min<int64_t>(cs.hub_n, hub_blocks);
             VT_LAUNCH(ctx, "local_move_hub", k_move_hbm, (unsigned)blocks, 64, 0, a, (const int32_t*)(lists.p + cs.hub_off),

@@ -194,16 +194,26 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
     for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
         const MemberRows mr(cstart, mem, moff, offs, c, lane);
         int nd = 0;  // distinct keys so far (the table never holds more than kCfList)
+        DEdge e{};   // this lane's entry of the current chunk
+        if (mr.s0 < mr.s1) {
+            const int64_t k = mr.csr_index(mr.s0 + lane, mem, moff, offs);
+            if (mr.s0 + lane < mr.s1) e = edges[k];
+        }
         for (int64_t base = mr.s0; base < mr.s1 && nd <= kCfMaxKeys; base += 64) {
             const int64_t p = base + lane;
-            const int64_t k = mr.csr_index(p, mem, moff, offs);
             const bool valid = p < mr.s1;
             uint32_t c2 = 0;
             double wt = -0.0;
             if (valid) {
-                const DEdge e = edges[k];
                 c2 = newid[e.tail];
                 wt = e.weight;
+            }
+            // the next chunk's entries are loaded while this one is folded (issued after the
+            // gather above, so waiting for the gather does not wait for them)
+            DEdge en{};
+            if (base + 64 < mr.s1) {
+                const int64_t kn = mr.csr_index(p + 64, mem, moff, offs);
+                if (p + 64 < mr.s1) en = edges[kn];
             }
             int slot = 0;
             bool fresh = false;
@@ -236,6 +246,7 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
                 tm[w][slot] = 0;
             }
             wave_lds_sync();
+            e = en;
         }
         if (nd <= kCfMaxKeys) {
             // each key's column is its rank among the (distinct) keys
