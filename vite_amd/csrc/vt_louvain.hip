@@ -1218,6 +1218,35 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(const int64_t* __restric
     }
 }
 
+// Long rows (the coarse levels): one wave per row loads 64 weights at a time with coalesced
+// reads (the next 64 in flight while these are added), stages them in LDS, and lane 0 adds
+// them in order -- the same left fold as k_row_fold.
+constexpr int kFoldWaveRows = 4;
+__global__ void __launch_bounds__(64 * kFoldWaveRows) k_row_fold_wave(const int64_t* __restrict__ offs,
+                                                                     const DEdge* __restrict__ edges, int64_t nv,
+                                                                     double* __restrict__ vdeg)
+{
+    __shared__ double st[kFoldWaveRows][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int64_t i = (int64_t)blockIdx.x * kFoldWaveRows + w; i < nv; i += (int64_t)gridDim.x * kFoldWaveRows) {
+        const int64_t e0 = offs[i], e1 = offs[i + 1];
+        double acc = 0.0;
+        double x = (e0 + lane < e1) ? edges[e0 + lane].weight : 0.0;
+        for (int64_t base = e0; base < e1; base += 64) {
+            const double xn = (base + 64 + lane < e1) ? edges[base + 64 + lane].weight : 0.0;
+            st[w][lane] = x;
+            wave_sync();
+            if (lane == 0) {
+                const int n = (int)(e1 - base < 64 ? e1 - base : 64);
+                for (int u = 0; u < n; u++) acc = acc + st[w][u];
+            }
+            wave_sync();
+            x = xn;
+        }
+        if (lane == 0) vdeg[i] = acc;
+    }
+}
+
 __global__ void k_init(int64_t nv, const double* __restrict__ vdeg, double* __restrict__ cdeg,
                        int64_t* __restrict__ csize, int64_t* __restrict__ past, int64_t* __restrict__ curr,
                        int64_t* __restrict__ target, unsigned char* __restrict__ active, double* __restrict__ frozen,
@@ -1315,23 +1344,50 @@ __global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, cons
     }
 }
 
+// Each sorted event's contribution, for all events at once: +k_v when v arrives in the
+// event's community, -k_v when it leaves (a - x == a + (-x) exactly), and +-1 to the member
+// count.  The per-community folds below then read them as a stream.
+__global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
+                               const int64_t* __restrict__ target, const double* __restrict__ vdeg, uint32_t none,
+                               double* __restrict__ delta, int8_t* __restrict__ dir)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = keys[j];
+        if (c == none) continue;
+        const uint32_t v = vals[j];
+        const bool in = (uint64_t)target[v] == (uint64_t)c;
+        const double k = vdeg[v];
+        delta[j] = in ? k : -k;
+        dir[j] = in ? 1 : -1;
+    }
+}
+
 // Events sorted stably by community keep processing order inside a community, so the
 // head of each run folds localCupdate exactly as one thread's sequence of `omp atomic`
 // updates does (+= k_i on arrival, -= k_i on departure), then localCinfo += localCupdate
 // (louvain.cpp:2534-2547).
-__global__ void k_fold_events(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                              const int64_t* __restrict__ target, const double* __restrict__ vdeg,
-                              double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none)
+__global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* __restrict__ delta,
+                              const int8_t* __restrict__ dir, int64_t n, double* __restrict__ cdeg,
+                              int64_t* __restrict__ csize, uint32_t none)
 {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
         if (c == none || (j > 0 && keys[j - 1] == c)) continue;
         double acc = 0.0;
         int64_t cnt = 0;
-        for (int64_t k = j; k < n && keys[k] == c; k++) {
-            const uint32_t v = vals[k];
-            if ((uint64_t)target[v] == (uint64_t)c) { acc = acc + vdeg[v]; cnt++; }
-            else { acc = acc - vdeg[v]; cnt--; }
+        int64_t k = j;
+        // keys ascend, so keys[k + 3] == c means keys[k..k+3] all are c
+        for (; k + 4 <= n && keys[k + 3] == c; k += 4) {
+            const double d0 = delta[k], d1 = delta[k + 1], d2 = delta[k + 2], d3 = delta[k + 3];
+            acc = acc + d0;
+            acc = acc + d1;
+            acc = acc + d2;
+            acc = acc + d3;
+            cnt += dir[k] + dir[k + 1] + dir[k + 2] + dir[k + 3];
+        }
+        for (; k < n && keys[k] == c; k++) {
+            acc = acc + delta[k];
+            cnt += dir[k];
         }
         cdeg[c] = cdeg[c] + acc;
         csize[c] += cnt;
@@ -1566,7 +1622,12 @@ struct Phase {
 
     void init_state(bool with_et)
     {
-        if (nv) VT_LAUNCH(ctx, "row_fold", k_row_fold, grid_for(nv, kFoldRows, 16384), kFoldRows, 0, g->offs, g->edges, nv, vdeg.p);
+        if (nv && nv < 65536 && g->ne > 48 * nv)  // few, long rows: a thread per row would leave the GPU idle
+            VT_LAUNCH(ctx, "row_fold", k_row_fold_wave, grid_for(nv, kFoldWaveRows, 65536), 64 * kFoldWaveRows, 0, g->offs,
+                      g->edges, nv, vdeg.p);
+        else if (nv)
+            VT_LAUNCH(ctx, "row_fold", k_row_fold, grid_for(nv, kFoldRows, 16384), kFoldRows, 0, g->offs, g->edges, nv,
+                      vdeg.p);
         VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, nv, (const double*)vdeg.p, cdeg.p, csize.p, past.p,
                   curr.p, target.p, with_et ? active.p : nullptr, with_et ? frozen.p : nullptr,
                   with_et ? pa.p : nullptr, with_et ? pb.p : nullptr);
@@ -1806,11 +1867,19 @@ struct Phase {
                   (const int32_t*)pos, target.p, curr.p, k0, v0);
         bool alt = false;
         sort_pairs_u32(ctx, k0, v0, ne, (int)bits_for(nv > 1 ? nv - 1 : 1), k1, v1, &alt);
-        const uint32_t* ks = alt ? k1 : k0;
-        const uint32_t* vs = alt ? v1 : v0;
-        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, target.p, vdeg.p,
-                  dst_cdeg, dst_csize, 0xFFFFFFFFu);
+        fold_events(alt ? k1 : k0, alt ? v1 : v0, ne, 0xFFFFFFFFu, (double*)scratch(ctx, "ev_d", 8 * (size_t)ne),
+                    (int8_t*)scratch(ctx, "ev_s", (size_t)ne), dst_cdeg, dst_csize);
         return movers;
+    }
+
+    // localCinfo += the sorted events' folds, per community in processing order.
+    void fold_events(const uint32_t* ks, const uint32_t* vs, int64_t ne, uint32_t none, double* delta, int8_t* dir,
+                     double* dst_cdeg, int64_t* dst_csize)
+    {
+        VT_LAUNCH(ctx, "event_deltas", k_event_deltas, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne,
+                  (const int64_t*)target.p, (const double*)vdeg.p, none, delta, dir);
+        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, (const double*)delta,
+                  (const int8_t*)dir, ne, dst_cdeg, dst_csize, none);
     }
 
     double modularity(int flags)
@@ -2014,6 +2083,8 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     uint32_t* v0 = (uint32_t*)scratch(ctx, "bev_v0", 4 * (size_t)nev);
     uint32_t* k1 = (uint32_t*)scratch(ctx, "bev_k1", 4 * (size_t)nev);
     uint32_t* v1 = (uint32_t*)scratch(ctx, "bev_v1", 4 * (size_t)nev);
+    double* dl = (double*)scratch(ctx, "bev_d", 8 * (size_t)nev);
+    int8_t* dr = (int8_t*)scratch(ctx, "bev_s", (size_t)nev);
     const uint32_t none = (uint32_t)nv;
     const int bits = (int)bits_for(std::max<int64_t>(nv, 1));
     bool alt = false;
@@ -2025,10 +2096,7 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     };
     auto sort = [&]() { sort_pairs_u32(ctx, k0, v0, 2 * nv, bits, k1, v1, &alt); };
     auto post = [&]() {
-        const uint32_t* ks = alt ? k1 : k0;
-        const uint32_t* vs = alt ? v1 : v0;
-        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(2 * nv, 256, 16384), 256, 0, ks, vs, 2 * nv,
-                  (const int64_t*)ph.target.p, (const double*)ph.vdeg.p, ph.cdeg.p, ph.csize.p, none);
+        ph.fold_events(alt ? k1 : k0, alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p);
         if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
         else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
         dzero(ctx, nact.p, 4);
