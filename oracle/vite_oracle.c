@@ -842,6 +842,12 @@ int or_gen_rgg(uint64_t seed, int64_t n, double extra_pct, or_graph* out)
         const int64_t m = out->ne / 2;
         const int64_t nx = ((int64_t)(extra_pct * (double)m)) / 100;
         xedge* acc = (xedge*)malloc(sizeof(xedge) * (size_t)(nx > 0 ? nx : 1));
+        /* accepted pairs, open addressing (a = -1: empty slot) */
+        int64_t cap = 2;
+        while (cap < 2 * nx + 2) cap <<= 1;
+        int64_t* ha = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+        int64_t* hb = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+        for (int64_t k = 0; k < cap; k++) ha[k] = -1;
         int64_t na = 0;
         uint64_t t = 0;
         while (na < nx) {
@@ -852,11 +858,19 @@ int or_gen_rgg(uint64_t seed, int64_t n, double extra_pct, or_graph* out)
             if (a0 == b0) continue;
             const int64_t a = a0 < b0 ? a0 : b0, b = a0 < b0 ? b0 : a0;
             if (has_edge(out, a, b)) continue;
+            uint64_t h = ((uint64_t)a * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)b * 0xC2B2AE3D27D4EB4Full);
+            int64_t slot = (int64_t)((h ^ (h >> 29)) & (uint64_t)(cap - 1));
             int dup = 0;
-            for (int64_t k = 0; k < na && !dup; k++) dup = (acc[k].a == a && acc[k].b == b);
+            while (ha[slot] != -1) {
+                if (ha[slot] == a && hb[slot] == b) { dup = 1; break; }
+                slot = (slot + 1) & (cap - 1);
+            }
             if (dup) continue;
+            ha[slot] = a;
+            hb[slot] = b;
             acc[na++] = (xedge){a, b, w, (int64_t)tt};
         }
+        free(ha); free(hb);
         int rc = merge_extra(out, acc, na);
         free(acc);
         if (rc) return rc;

@@ -1,8 +1,14 @@
 """GPU parity at larger sizes: a 2^20-vertex RGG (29 M directed entries) swept from a
-mid-run device state, and a hub-heavy Barabasi-Albert graph (2^17 vertices, m = 8, rows up
-to a few thousand entries, so the sorted hub path runs) through whole phases and runs.
-The graphs are generated on the device (the generators' parity is tested at smaller sizes)
-and downloaded for the oracle."""
+mid-run device state, a hub-heavy Barabasi-Albert graph (2^17 vertices, m = 8, rows up to
+a few thousand entries, so the sorted hub path runs) through whole phases and runs, and the
+benchmark's two RGGs (n = 65,536 and n = 2^24 + 1%) through whole runs against the compiled
+reference's main() (tests/golden/reference_large.json).  The graphs are generated on the
+device (the generators' parity is tested at smaller sizes; the benchmark graphs' row offsets
+are checked against the vectors' hashes) and downloaded for the oracle where it is used."""
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -58,3 +64,49 @@ def test_ba_hubs_run_exact(ctx, ba_hubs):
     assert r.call_q == r_o.call_q
     assert r.call_iters == r_o.call_iters
     assert r.level_nv == r_o.level_nv
+
+
+LARGE = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_large.json")))["cases"]
+
+
+def _offs_sha(dg):
+    return hashlib.sha256(np.ascontiguousarray(dg.offsets(), np.int64).tobytes()).hexdigest()
+
+
+def test_rgg_65536_run_matches_reference(ctx):
+    """The benchmark's CPU-runnable case (RGG n = 65,536, threshold 1e-6) through the whole
+    main.cpp loop on the device.  With sequential sums every call's value and iteration count
+    equal the reference's one-thread run; with the default tree sums the final modularity is
+    within 1e-6 of it."""
+    case = LARGE["rgg_65536"]
+    spec = case["spec"]
+    dg = ctx.gen_rgg(spec["seed"], spec["n"], spec["extra_pct"])
+    assert (dg.nv, dg.ne) == (case["nv"], case["ne"])
+    assert _offs_sha(dg) == case["offs_sha256"]
+    r = dg.run(sequential=True)
+    assert r.call_q == [float.fromhex(c["q_hex"]) for c in case["calls"]]
+    assert r.call_iters == [c["iters"] for c in case["calls"]]
+    assert r.level_nv[1:] == [rb["nv"] for rb in case["rebuilds"]][: len(r.level_nv) - 1]
+    assert r.level_ne[1:] == [rb["ne"] for rb in case["rebuilds"]][: len(r.level_ne) - 1]
+    t = dg.run()
+    assert abs(t.final_q - case["calls"][-1]["q"]) <= 1e-6
+    dg.free()
+
+
+@pytest.mark.skipif("rgg_2p24_i" not in LARGE, reason="no reference vectors for the 2^24 RGG")
+def test_rgg_2p24_threshold_cycling_matches_reference(ctx):
+    """The benchmark graph (RGG n = 2^24 + 1% extra edges, 563 M directed entries) with
+    threshold cycling: every call's modularity within 1e-6 of the reference main() run on
+    the same graph (its OpenMP sums at several threads fix their own order, so equality is
+    not expected), the same number of calls, and the final modularity within 1e-6."""
+    case = LARGE["rgg_2p24_i"]
+    spec = case["spec"]
+    dg = ctx.gen_rgg(spec["seed"], spec["n"], spec["extra_pct"])
+    assert (dg.nv, dg.ne) == (case["nv"], case["ne"])
+    assert _offs_sha(dg) == case["offs_sha256"]
+    r = dg.run(threshold_cycling=True)
+    ref = [c["q"] for c in case["calls"]]
+    assert len(r.call_q) == len(ref)
+    assert max(abs(a - b) for a, b in zip(r.call_q, ref)) <= 1e-6
+    assert abs(r.final_q - ref[-1]) <= 1e-6
+    dg.free()

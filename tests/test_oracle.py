@@ -98,6 +98,26 @@ def test_golden_runs(golden_graphs, idx):
     assert [c["nv"] for c in run["calls"][: len(r.level_nv)]] == r.level_nv[: len(run["calls"])]
 
 
+LARGE = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_large.json")))["cases"]
+
+
+def test_golden_large_rgg_65536():
+    """The benchmark's CPU-runnable case (RGG n = 65,536, threshold 1e-6): the oracle's phase
+    loop reproduces every call of the reference main() at one thread bit for bit."""
+    case = LARGE["rgg_65536"]
+    spec = case["spec"]
+    g = po.gen_rgg(spec["seed"], spec["n"], spec["extra_pct"])
+    assert (g.nv, g.ne) == (case["nv"], case["ne"])
+    assert hashlib.sha256(np.ascontiguousarray(g.offs, np.int64).tobytes()).hexdigest() == case["offs_sha256"]
+    r = po.run(g, **flags_to_kwargs(spec["flags"]))
+    assert [float.fromhex(c["q_hex"]) for c in case["calls"]] == r.call_q
+    assert [c["iters"] for c in case["calls"]] == r.call_iters
+    assert [c["cvect_fnv"] for c in case["calls"]] == [f"{x:016x}" for x in r.call_cvect_fnv]
+    # level k + 1 is the graph rebuild k produced (the last phase's graph is not a level)
+    assert r.level_nv[1:] == [rb["nv"] for rb in case["rebuilds"]][: len(r.level_nv) - 1]
+    assert r.level_ne[1:] == [rb["ne"] for rb in case["rebuilds"]][: len(r.level_ne) - 1]
+
+
 @pytest.mark.parametrize("idx", range(len(GOLDEN["sweeps"])))
 def test_golden_sweeps(golden_graphs, idx):
     s = GOLDEN["sweeps"][idx]

@@ -100,11 +100,15 @@ __global__ void k_iota_u32(uint32_t* __restrict__ p, int64_t n)
         p[i] = (uint32_t)i;
 }
 
+// Each member's row length (for the scan into concatenated-row offsets) and row start.
 __global__ void k_member_degrees(const uint32_t* __restrict__ mem, int64_t nv, const int64_t* __restrict__ offs,
-                                 int64_t* __restrict__ mdeg)
+                                 int64_t* __restrict__ mdeg, int64_t* __restrict__ mrow)
 {
-    for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m <= nv; m += (int64_t)gridDim.x * blockDim.x)
-        mdeg[m] = m < nv ? offs[mem[m] + 1] - offs[mem[m]] : 0;
+    for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m <= nv; m += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r0 = m < nv ? offs[mem[m]] : 0;
+        mdeg[m] = m < nv ? offs[mem[m] + 1] - r0 : 0;
+        if (m < nv) mrow[m] = r0;
+    }
 }
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -114,15 +118,25 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Positions of community c's concatenated row -> CSR indices.  Up to 64 members, lane i
-// holds member i's start in the concatenated row and its row start in the CSR, and lanes
-// find their member with shuffles; larger communities search moff.
+__device__ __forceinline__ double readlane_f64(double x, int j)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Positions of community c's concatenated row -> CSR indices (moff: each member's start in
+// its community's concatenated row; mrow: its row start in the CSR).  Up to 64 members,
+// lane i holds member i's two starts and lanes find their member with shuffles; larger
+// communities search moff.  c is the same in every lane, so the community's bounds come in
+// through scalar loads.
 struct MemberRows {
     int64_t m0, m1, s0, s1, my_s = INT64_MAX, my_row = 0;
     bool small;
 
-    __device__ MemberRows(const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem,
-                          const int64_t* __restrict__ moff, const int64_t* __restrict__ offs, int64_t c, int lane)
+    __device__ MemberRows(const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff,
+                          const int64_t* __restrict__ mrow, int64_t c, int lane)
     {
         m0 = cstart[c];
         m1 = cstart[c + 1];
@@ -131,13 +145,12 @@ struct MemberRows {
         small = m1 - m0 <= 64;
         if (small && lane < m1 - m0) {
             my_s = moff[m0 + lane];
-            my_row = offs[mem[m0 + lane]];
+            my_row = mrow[m0 + lane];
         }
     }
 
     // CSR index of position p (p < s1); every lane of the wave calls it.
-    __device__ int64_t csr_index(int64_t p, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
-                                 const int64_t* __restrict__ offs) const
+    __device__ int64_t csr_index(int64_t p, const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow) const
     {
         if (small) {
             const int nm = (int)(m1 - m0);
@@ -157,7 +170,7 @@ struct MemberRows {
             if (moff[mid] <= p) lo = mid;
             else hi = mid - 1;
         }
-        return offs[mem[lo]] + (p - moff[lo]);
+        return mrow[lo] + (p - moff[lo]);
     }
 };
 
@@ -172,8 +185,8 @@ constexpr uint32_t kNoKey = 0xffffffffu;
 // offset in the concatenated rows; or, with more distinct keys than the table holds,
 // rowlen[c] = 0 and xcnt[c] = its entry count, for the sort path.
 __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
-    int64_t nc, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
-    const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
+    int64_t nc, const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow,
+    const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
     uint32_t* __restrict__ rkey, double* __restrict__ rval, int64_t* __restrict__ rowlen, int64_t* __restrict__ xcnt)
 {
     __shared__ uint32_t tk[kCfWaves][kCfCap];
@@ -182,7 +195,7 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
     __shared__ uint32_t tl[kCfWaves][kCfList];  // keys in insertion order
     __shared__ uint16_t ts[kCfWaves][kCfList];  // and their slots
     __shared__ double wst[kCfWaves][65];        // [0] = -0.0, then the chunk's weights
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned long long below = (1ull << lane) - 1;
     for (int i = lane; i < kCfCap; i += 64) {
         tk[w][i] = kNoKey;
@@ -192,11 +205,13 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
     if (lane == 0) wst[w][0] = -0.0;
     wave_lds_sync();
     for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
-        const MemberRows mr(cstart, mem, moff, offs, c, lane);
-        int nd = 0;  // distinct keys so far (the table never holds more than kCfList)
-        DEdge e{};   // this lane's entry of the current chunk
+        const MemberRows mr(cstart, moff, mrow, c, lane);
+        int nd = 0;  // distinct external keys so far (the table never holds more than kCfList)
+        double self = -0.0;  // the internal entries' fold, the same in every lane
+        bool any_self = false;
+        DEdge e{};  // this lane's entry of the current chunk
         if (mr.s0 < mr.s1) {
-            const int64_t k = mr.csr_index(mr.s0 + lane, mem, moff, offs);
+            const int64_t k = mr.csr_index(mr.s0 + lane, moff, mrow);
             if (mr.s0 + lane < mr.s1) e = edges[k];
         }
         for (int64_t base = mr.s0; base < mr.s1 && nd <= kCfMaxKeys; base += 64) {
@@ -212,12 +227,19 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
             // gather above, so waiting for the gather does not wait for them)
             DEdge en{};
             if (base + 64 < mr.s1) {
-                const int64_t kn = mr.csr_index(p + 64, mem, moff, offs);
+                const int64_t kn = mr.csr_index(p + 64, moff, mrow);
                 if (p + 64 < mr.s1) en = edges[kn];
             }
+            // internal entries (about half after a phase) form one long run per chunk: add
+            // them in lane order with the lane read as a scalar, without the table
+            const bool in = valid && c2 == (uint32_t)c;
+            const bool ext = valid && !in;
+            const unsigned long long im = __ballot(in);
+            any_self = any_self || im != 0;
+            for (unsigned long long mm = im; mm; mm &= mm - 1) self = self + readlane_f64(wt, __builtin_ctzll(mm));
             int slot = 0;
             bool fresh = false;
-            if (valid) {
+            if (ext) {
                 uint32_t h = (c2 * 0x9E3779B1u) >> (32 - kCfLog);
                 for (;;) {  // ends: at most kCfList < kCfCap keys are ever present
                     const uint32_t old = atomicCAS(&tk[w][h], kNoKey, c2);
@@ -237,11 +259,11 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
             }
             nd += __popcll(fm);
             wst[w][1 + lane] = wt;
-            if (valid) atomicOr(&tm[w][slot], 1ull << lane);
+            if (ext) atomicOr(&tm[w][slot], 1ull << lane);
             wave_lds_sync();
-            const unsigned long long m = valid ? tm[w][slot] : 0ull;
+            const unsigned long long m = ext ? tm[w][slot] : 0ull;
             wave_lds_sync();
-            if (valid && (m & below) == 0) {
+            if (ext && (m & below) == 0) {
                 tv[w][slot] = fold_rest<unsigned long long, 2>(tv[w][slot], m, &wst[w][1]);
                 tm[w][slot] = 0;
             }
@@ -249,16 +271,27 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
             e = en;
         }
         if (nd <= kCfMaxKeys) {
-            // each key's column is its rank among the (distinct) keys
-            for (int i = lane; i < nd; i += 64) {
-                const uint32_t ki = tl[w][i];
-                int rank = 0;
-                for (int j = 0; j < nd; j++) rank += tl[w][j] < ki ? 1 : 0;
-                rkey[mr.s0 + rank] = ki;
-                rval[mr.s0 + rank] = tv[w][ts[w][i]];
+            // each external key's column is its rank among the (distinct) keys, one further
+            // when the self-loop entry sorts before it
+            int lt = 0;  // external keys below c
+            for (int i0 = 0; i0 < nd; i0 += 64) {
+                const int i = i0 + lane;
+                const uint32_t ki = i < nd ? tl[w][i] : kNoKey;
+                lt += __popcll(__ballot(ki < (uint32_t)c));
+                if (i < nd) {
+                    int rank = 0;
+                    for (int j = 0; j < nd; j++) rank += tl[w][j] < ki ? 1 : 0;
+                    const int64_t q = mr.s0 + rank + ((any_self && ki > (uint32_t)c) ? 1 : 0);
+                    rkey[q] = ki;
+                    rval[q] = tv[w][ts[w][i]];
+                }
             }
             if (lane == 0) {
-                rowlen[c] = nd;
+                if (any_self) {
+                    rkey[mr.s0 + lt] = (uint32_t)c;
+                    rval[mr.s0 + lt] = self;
+                }
+                rowlen[c] = nd + (any_self ? 1 : 0);
                 xcnt[c] = 0;
             }
         } else if (lane == 0) {
@@ -278,19 +311,19 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
 // Every entry of the communities that overflowed the table, as ((C1 << bits) | C2, weight),
 // at xoff[C1] + its position in C1's concatenated row.
 __global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit(
-    int64_t nc, const int64_t* __restrict__ cstart, const uint32_t* __restrict__ mem, const int64_t* __restrict__ moff,
-    const int64_t* __restrict__ offs, const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
+    int64_t nc, const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow,
+    const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
     const int64_t* __restrict__ xcnt, const int64_t* __restrict__ xoff, int bits, unsigned long long* __restrict__ keys,
     double* __restrict__ vals)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
         if (xcnt[c] == 0) continue;
-        const MemberRows mr(cstart, mem, moff, offs, c, lane);
+        const MemberRows mr(cstart, moff, mrow, c, lane);
         const int64_t o = xoff[c] - mr.s0;
         for (int64_t base = mr.s0; base < mr.s1; base += 64) {
             const int64_t p = base + lane;
-            const int64_t k = mr.csr_index(p, mem, moff, offs);
+            const int64_t k = mr.csr_index(p, moff, mrow);
             if (p < mr.s1) {
                 const DEdge e = edges[k];
                 keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
@@ -393,14 +426,14 @@ __global__ void k_pair_totals(const int64_t* __restrict__ a, const int64_t* __re
 // Overflowing communities: their entries sorted stably by (C1, C2), runs folded left to
 // right, and each community's runs placed in its staged row.
 static void overflow_rows(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid, int64_t nc, int bits,
-                          const int64_t* cstart, const uint32_t* mem, const int64_t* moff, const int64_t* xcnt,
+                          const int64_t* cstart, const int64_t* mrow, const int64_t* moff, const int64_t* xcnt,
                           const int64_t* xoff, int64_t nx, uint32_t* rkey, double* rval, int64_t* rowlen)
 {
     const size_t x1 = (size_t)std::max<int64_t>(nx, 1), c1n = (size_t)nc + 1;
     DBuf<unsigned long long> k0(ctx, x1), k1(ctx, x1);
     DBuf<double> v0(ctx, x1), v1(ctx, x1);
-    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, mem, moff,
-              (const int64_t*)g->offs, (const DEdge*)g->edges, newid, xcnt, xoff, bits, k0.p, v0.p);
+    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, moff, mrow,
+              (const DEdge*)g->edges, newid, xcnt, xoff, bits, k0.p, v0.p);
     const unsigned long long* ks = k0.p;
     const double* vs = v0.p;
     if (nx > 1) {
@@ -447,8 +480,7 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
 
     // members of each community in ascending vertex order (stable sort by new id), and each
     // member's start in its community's concatenated row
-    DBuf<int64_t> cstart(ctx, c1n), moff(ctx, n1 + 1);
-    DBuf<uint32_t> mem;
+    DBuf<int64_t> cstart(ctx, c1n), moff(ctx, n1 + 1), mrow(ctx, n1);
     {
         DBuf<int64_t> ccnt(ctx, c1n);
         dzero(ctx, ccnt.p, 8 * c1n);
@@ -459,10 +491,9 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
         VT_LAUNCH(ctx, "rb_iota", k_iota_u32, grid_for(nv, 256, 8192), 256, 0, mv.p, nv);
         bool malt = false;
         sort_pairs_u32(ctx, mk.p, mv.p, nv, bits, mk2.p, mv2.p, &malt);
-        mem = std::move(malt ? mv2 : mv);
         DBuf<int64_t> mdeg(ctx, n1 + 1);
-        VT_LAUNCH(ctx, "rb_mdeg", k_member_degrees, grid_for(nv + 1, 256, 8192), 256, 0, (const uint32_t*)mem.p, nv,
-                  (const int64_t*)g->offs, mdeg.p);
+        VT_LAUNCH(ctx, "rb_mdeg", k_member_degrees, grid_for(nv + 1, 256, 8192), 256, 0,
+                  (const uint32_t*)(malt ? mv2.p : mv.p), nv, (const int64_t*)g->offs, mdeg.p, mrow.p);
         exclusive_scan_i64(ctx, mdeg.p, moff.p, nv + 1);
     }
 
@@ -474,8 +505,8 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     dzero(ctx, xcnt.p + nc, 8);
     if (nc)
         VT_LAUNCH(ctx, "rb_comm_fold", k_comm_fold, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc,
-                  (const int64_t*)cstart.p, (const uint32_t*)mem.p, (const int64_t*)moff.p, (const int64_t*)g->offs,
-                  (const DEdge*)g->edges, newid, rkey.p, rval.p, rowlen.p, xcnt.p);
+                  (const int64_t*)cstart.p, (const int64_t*)moff.p, (const int64_t*)mrow.p, (const DEdge*)g->edges, newid,
+                  rkey.p, rval.p, rowlen.p, xcnt.p);
     exclusive_scan_i64(ctx, rowlen.p, out_offs.p, nc + 1);
     exclusive_scan_i64(ctx, xcnt.p, xoff.p, nc + 1);
     VT_LAUNCH(ctx, "rb_totals", k_pair_totals, 1, 64, 0, (const int64_t*)out_offs.p + nc, (const int64_t*)xoff.p + nc,
@@ -483,7 +514,7 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     int64_t totals[2] = {0, 0};
     d2h(ctx, totals, tot.p, 16);
     if (totals[1]) {
-        overflow_rows(ctx, g, newid, nc, bits, cstart.p, mem.p, moff.p, xcnt.p, xoff.p, totals[1], rkey.p, rval.p,
+        overflow_rows(ctx, g, newid, nc, bits, cstart.p, mrow.p, moff.p, xcnt.p, xoff.p, totals[1], rkey.p, rval.p,
                       rowlen.p);
         exclusive_scan_i64(ctx, rowlen.p, out_offs.p, nc + 1);
         d2h(ctx, totals, out_offs.p + nc, 8);
