@@ -106,6 +106,22 @@ def summarize_profile(d):
     return "\n".join(lines) + "\n"
 
 
+def reference_run(args):
+    """The reference main() run on the benchmark graph, when the committed vectors cover this
+    exact input (seed, n, extra-edge percentage): per-call Q, iterations and wall time."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "reference_large.json")
+    try:
+        cases = json.load(open(path))["cases"]
+    except (OSError, ValueError, KeyError):
+        return None
+    for case in cases.values():
+        sp = case["spec"]
+        if (sp["seed"], sp["n"], sp["extra_pct"], sp["flags"]) == (args.seed, args.n, args.extra, ["-i"]):
+            return {"q": [c["q"] for c in case["calls"]], "iters": [c["iters"] for c in case["calls"]],
+                    "seconds": case.get("reference_seconds"), "threads": sp["threads"]}
+    return None
+
+
 def batched_form(ctx, args):
     """The batched form: args.batch_graphs seeded RGGs, n log-uniform in [2^8, 2^16], every
     other graph on average with 1% extra edges; all run through the threshold-cycling loop
@@ -394,6 +410,14 @@ def main():
         out["time_to_final_q"] = {"seconds": ttq, "final_q": r.final_q, "levels": len(r.level_q),
                                   "calls": len(r.call_q), "total_iters": r.total_iters,
                                   "clustering_seconds": r.clustering_seconds, "flags": "-i"}
+        ref = reference_run(args)
+        if ref:
+            dq = max(abs(a - b) for a, b in zip(r.call_q, ref["q"])) if len(ref["q"]) == len(r.call_q) else None
+            out["time_to_final_q"]["reference"] = {
+                "final_q": ref["q"][-1], "abs_diff_final_q": abs(r.final_q - ref["q"][-1]), "max_abs_diff_call_q": dq,
+                "same_iterations": ref["iters"] == r.call_iters, "seconds": ref["seconds"],
+                "source": "the reference's main() -i on the same graph, 1 MPI rank, run in the build container "
+                          f"({ref['threads']} OpenMP threads; not the GPU host)"}
         log(f"time to final Q: {ttq:.2f}s q={r.final_q:.9f} levels={len(r.level_q)} iters={r.total_iters}")
 
     if args.skewed_n > 0:

@@ -107,6 +107,12 @@ def test_rgg_2p24_threshold_cycling_matches_reference(ctx):
     r = dg.run(threshold_cycling=True)
     ref = [c["q"] for c in case["calls"]]
     assert len(r.call_q) == len(ref)
-    assert max(abs(a - b) for a, b in zip(r.call_q, ref)) <= 1e-6
+    dq = max(abs(a - b) for a, b in zip(r.call_q, ref))
+    print(f"max |dQ| over {len(ref)} calls: {dq:.3e}; final Q {r.final_q!r} against {ref[-1]!r}")
+    assert dq <= 1e-6
     assert abs(r.final_q - ref[-1]) <= 1e-6
+    # the same phases: iteration counts and coarse graph sizes
+    assert r.call_iters == [c["iters"] for c in case["calls"]]
+    assert r.level_nv[1:] == [rb["nv"] for rb in case["rebuilds"]][: len(r.level_nv) - 1]
+    assert r.level_ne[1:] == [rb["ne"] for rb in case["rebuilds"]][: len(r.level_ne) - 1]
     dg.free()
