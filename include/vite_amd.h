@@ -246,7 +246,8 @@ int vt_batch_free(vt_ctx* ctx, vt_batch* b);
  * graph on the device and runs the phase loop on it.  Requests are admitted in arrival
  * order when a slot is free and their HBM reservation (vt_request_reservation) fits beside
  * those of the running jobs; a job cannot allocate beyond its reservation.  A result does
- * not depend on what else runs: every job is computed alone on its own stream. */
+ * not depend on what else runs: every job runs on its own stream, and a batched job gives
+ * each graph what it gets alone. */
 #define VT_REQ_RGG 0  /* seeded RGG (vt_gen_rgg): param = extra edge percent         */
 #define VT_REQ_BA 1   /* seeded Barabasi-Albert (vt_gen_ba): param = attachments m    */
 typedef struct {
@@ -265,12 +266,20 @@ typedef struct {
     uint64_t reserved_bytes;  /* HBM reserved at admission                                 */
     uint64_t peak_bytes;      /* HBM high-water mark of the job                            */
     int64_t nv, ne;           /* size of the generated graph                               */
+    int32_t batch;            /* requests run together in its job (1: alone)               */
 } vt_request_info;
 typedef struct vt_server vt_server;
 /* Bytes the server reserves for a request (an upper bound on what its job allocates). */
 uint64_t vt_request_reservation(const vt_request* req);
 int vt_server_create(int device, uint64_t hbm_budget_bytes, int32_t slots, vt_server** out);
 /* Queues a request; VT_ERR_BUDGET if its reservation exceeds the whole budget. */
+/* Small-request batching: a slot that admits a request of at most max_n vertices (plain
+ * Louvain: no coloring, vertex order or early termination) also takes up to max_batch - 1
+ * such requests with the same run parameters queued right behind it, as far as their
+ * reservations fit, and runs them as one batched job (vt_batch_run); each result equals the
+ * request's vt_run alone.  The batched job reserves twice its members' reservations (their
+ * graphs and the union).  Default: max_batch 32, max_n 65536; max_batch 1 turns it off. */
+int vt_server_set_batching(vt_server* s, int32_t max_batch, int64_t max_n);
 int vt_server_submit(vt_server* s, const vt_request* req, int64_t* ticket);
 /* Blocks until the request has finished; returns its status.  Each ticket is waited once. */
 int vt_server_wait(vt_server* s, int64_t ticket, vt_run_result* res, vt_request_info* info);

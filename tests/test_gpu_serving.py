@@ -1,9 +1,10 @@
 """GPU: the serving form (vt_server_*).
 
-Each request's result is the same alone and in the mix (exact: every job runs alone on its
-own stream with the same kernels), equals vt_run on the same generated graph, and the
-admission rule holds: arrival order, and running reservations never above the budget, with
-every job's measured HBM high-water mark inside its reservation.
+Each request's result is the same alone and in the mix (exact: every job runs on its own
+stream, and a batched job of small requests gives each graph what vt_run gives it alone),
+equals vt_run on the same generated graph, and the admission rule holds: arrival order, and
+running reservations never above the budget, with every job's measured HBM high-water mark
+inside its reservation.
 """
 import itertools
 
@@ -68,11 +69,46 @@ def test_admission_order_and_budget(ctx):
     for i in infos:
         assert i["status"] == 0
         assert i["peak_bytes"] <= i["reserved_bytes"]
-    # at every job start, the jobs running then fit in the budget together
-    for i in infos:
-        t = i["start_s"]
-        running = [j for j in infos if j["start_s"] <= t < j["end_s"]]
+    # at every job start, the jobs running then fit in the budget together (the members of
+    # a batched job share one slot, start and reservation)
+    jobs = {(i["slot"], i["start_s"]): i for i in infos}
+    for t in [j["start_s"] for j in jobs.values()]:
+        running = [j for j in jobs.values() if j["start_s"] <= t < j["end_s"]]
         assert sum(j["reserved_bytes"] for j in running) <= budget
+
+
+@pytest.mark.parametrize("kw", RUN_KW, ids=["cycling", "plain"])
+def test_queued_small_requests_batched(ctx, kw):
+    """Small requests queued behind a running one are taken together as one batched job,
+    and every result still equals vt_run on that graph alone; larger requests and requests
+    with other run parameters are not batched."""
+    reqs = _requests(seed=13, count=20)
+    with vite_amd.Server(slots=1, max_batch=6, batch_max_n=3000) as srv:
+        tickets = [srv.submit(*r, **kw) for r in reqs]
+        out = [srv.wait(t) for t in tickets]
+        # a request with other run parameters, queued between batchable ones
+        t1 = srv.submit(*reqs[0], **kw)
+        t2 = srv.submit(*reqs[1], threshold=1e-4)
+        t3 = srv.submit(*reqs[2], **kw)
+        r1, r2, r3 = srv.wait(t1), srv.wait(t2), srv.wait(t3)
+    assert max(i["batch"] for _, i in out) > 1
+    assert all(i["batch"] <= 6 for _, i in out)
+    for (kind, seed, n, p), (r, i) in zip(reqs, out):
+        assert i["status"] == 0
+        if n > 3000:
+            assert i["batch"] == 1
+        g = ctx.gen_rgg(seed, n, p) if kind == "rgg" else ctx.gen_ba(seed, n, p)
+        _same(r, g.run(**kw))
+        g.free()
+    _same(r1[0], out[0][0])
+    assert r2[1]["start_s"] != r1[1]["start_s"] and r2[1]["start_s"] != r3[1]["start_s"]
+
+
+def test_batching_settings_rejected(ctx):
+    with vite_amd.Server(slots=1) as srv:
+        for mb, mn in [(0, 100), (5000, 100), (4, 0)]:
+            with pytest.raises(vite_amd.VtError):
+                srv.set_batching(mb, mn)
 
 
 def test_oversized_request_rejected(ctx):

@@ -79,7 +79,7 @@ class _Request(C.Structure):
 class _RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("slot", C.c_int32), ("submit_s", C.c_double), ("start_s", C.c_double),
                 ("end_s", C.c_double), ("reserved_bytes", C.c_uint64), ("peak_bytes", C.c_uint64),
-                ("nv", C.c_int64), ("ne", C.c_int64)]
+                ("nv", C.c_int64), ("ne", C.c_int64), ("batch", C.c_int32)]
 
 
 class _BatchStats(C.Structure):
@@ -125,6 +125,7 @@ _SIGS = {
     "vt_request_reservation": (C.c_uint64, [C.POINTER(_Request)]),
     "vt_server_create": (C.c_int, [C.c_int, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     "vt_server_submit": (C.c_int, [C.c_void_p, C.POINTER(_Request), C.POINTER(C.c_int64)]),
+    "vt_server_set_batching": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "vt_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(_RunResult), C.POINTER(_RequestInfo)]),
     "vt_server_last_error": (C.c_char_p, [C.c_void_p]),
     "vt_server_destroy": (C.c_int, [C.c_void_p]),
@@ -456,12 +457,19 @@ class Server:
     """Serving loop (vt_server_*): requests admitted in arrival order by slot and HBM
     budget, run concurrently on per-slot streams."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 200 << 30, slots: int = 4):
+    def __init__(self, device: int = 0, hbm_budget: int = 200 << 30, slots: int = 4, max_batch: int = None,
+                 batch_max_n: int = None):
         h = C.c_void_p()
         rc = _lib.vt_server_create(device, hbm_budget, slots, C.byref(h))
         if rc != VT_OK:
             raise VtError(rc, "vt_server_create failed")
         self.h = h
+        if max_batch is not None or batch_max_n is not None:
+            self.set_batching(32 if max_batch is None else max_batch, 65536 if batch_max_n is None else batch_max_n)
+
+    def set_batching(self, max_batch: int, max_n: int):
+        """Small requests queued together run as one batched job (max_batch 1: never)."""
+        self._check(_lib.vt_server_set_batching(self.h, max_batch, max_n))
 
     def _check(self, rc):
         if rc != VT_OK:

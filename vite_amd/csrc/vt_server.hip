@@ -5,6 +5,11 @@
 // order, as soon as a slot is free and the request's HBM reservation fits beside the
 // reservations of the running jobs; a job cannot allocate beyond its reservation (its
 // context budget), so the running jobs together never exceed the server's budget.
+//
+// Small requests are batched: a slot that admits a small plain-Louvain request also takes
+// the small requests with the same run parameters queued right behind it (up to a batch
+// size, and while their reservation fits) and runs them as one batched job (vt_batch_*),
+// whose per-graph results equal vt_run's on each graph alone.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -25,6 +30,7 @@ namespace {
 
 struct Job {
     int64_t ticket = 0;
+    vt_graph* g = nullptr;
     vt_request req{};
     uint64_t reserve = 0;
     bool done = false;
@@ -46,6 +52,8 @@ struct vt_server {
     std::map<int64_t, std::unique_ptr<Job>> jobs;
     int64_t next_ticket = 1;
     uint64_t reserved = 0;
+    int32_t max_batch = 32;          // requests per batched job (1: no batching)
+    int64_t batch_max_n = 1 << 16;   // largest graph a batched job takes
     bool stop = false;
     std::string err;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -63,6 +71,77 @@ double expected_entries(const vt_request& r)
     const double n = (double)r.n;
     if (r.kind == VT_REQ_BA) return 2.0 * n * r.param;
     return n > 1 ? 2.0 * n * std::log(n) * (1.0 + r.param / 100.0) : 0.0;
+}
+
+// Plain Louvain with no coloring or early termination, small enough to batch.
+bool batchable(const vt_server* s, const vt_request& r)
+{
+    const vt_run_params& p = r.run;
+    return s->max_batch > 1 && r.n <= s->batch_max_n && p.coloring == 0 && p.vertex_order == 0 && p.et_type == 0 &&
+           !(p.one_phase && p.threshold_cycling);
+}
+
+// Run parameters a batch shares (the rest are unused for batchable requests).
+bool same_run(const vt_run_params& a, const vt_run_params& b)
+{
+    return a.threshold == b.threshold && a.threshold_cycling == b.threshold_cycling && a.one_phase == b.one_phase &&
+           a.flags == b.flags;
+}
+
+int gen_graph(vt_ctx* ctx, Job* j)
+{
+    const int rc = j->req.kind == VT_REQ_BA ? vt_gen_ba(ctx, j->req.seed, j->req.n, (int64_t)j->req.param, &j->g)
+                                            : vt_gen_rgg(ctx, j->req.seed, j->req.n, j->req.param, &j->g);
+    if (rc == VT_OK) vt_graph_info(j->g, &j->info.nv, &j->info.ne);
+    return rc;
+}
+
+// Several requests as one batched job: every graph generated on the slot, the ones with
+// edges run together through vt_batch_run (an edgeless graph goes through vt_run alone).
+void run_group(vt_server* s, int slot, const std::vector<Job*>& grp, uint64_t reserve)
+{
+    vt_ctx* ctx = s->ctxs[slot];
+    ctx->budget = reserve;
+    ctx->peak = ctx->in_use;
+    std::vector<Job*> members;
+    for (Job* j : grp) {
+        const int rc = gen_graph(ctx, j);
+        if (rc != VT_OK) {
+            j->info.status = rc;
+            j->err = vt_last_error(ctx);
+        } else if (j->info.ne > 0) {
+            members.push_back(j);
+        } else {
+            j->info.status = vt_run(ctx, j->g, &j->req.run, &j->res);
+            if (j->info.status != VT_OK) j->err = vt_last_error(ctx);
+        }
+    }
+    if (!members.empty()) {
+        std::vector<vt_graph*> gs;
+        for (Job* j : members) gs.push_back(j->g);
+        std::vector<vt_run_result> res(members.size());
+        vt_batch* b = nullptr;
+        int rc = vt_batch_create(ctx, (int32_t)gs.size(), gs.data(), &b);
+        if (rc == VT_OK) rc = vt_batch_run(ctx, b, &members[0]->req.run, res.data(), nullptr);
+        const std::string err = rc == VT_OK ? std::string() : std::string(vt_last_error(ctx));
+        if (b) vt_batch_free(ctx, b);
+        for (size_t i = 0; i < members.size(); i++) {
+            members[i]->info.status = rc;
+            members[i]->err = err;
+            if (rc == VT_OK) members[i]->res = res[i];
+        }
+    }
+    for (Job* j : grp) {
+        if (j->g) vt_graph_free(ctx, j->g);
+        j->g = nullptr;
+    }
+    release_scratch(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (Job* j : grp) {
+        j->info.peak_bytes = ctx->peak;  // the batched job's high-water mark
+        j->info.batch = (int32_t)grp.size();
+    }
+    ctx->budget = 0;
 }
 
 void run_job(vt_server* s, int slot, Job* j)
@@ -84,6 +163,7 @@ void run_job(vt_server* s, int slot, Job* j)
     (void)hipStreamSynchronize(ctx->stream);
     j->info.status = rc;
     j->info.peak_bytes = ctx->peak;
+    j->info.batch = 1;
     ctx->budget = 0;
 }
 
@@ -91,7 +171,8 @@ void worker(vt_server* s, int slot)
 {
     (void)hipSetDevice(s->device);
     for (;;) {
-        Job* j = nullptr;
+        std::vector<Job*> grp;
+        uint64_t reserve = 0;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
@@ -99,19 +180,43 @@ void worker(vt_server* s, int slot)
                        (!s->queue.empty() && s->reserved + s->queue.front()->reserve <= s->budget);
             });
             if (s->queue.empty()) return;  // stopping and drained
-            j = s->queue.front();
+            Job* j = s->queue.front();
             s->queue.pop_front();
-            s->reserved += j->reserve;
-            j->info.start_s = s->now();
-            j->info.slot = slot;
+            grp.push_back(j);
+            reserve = j->reserve;
+            if (batchable(s, j->req)) {
+                // the requests right behind it, while they match and fit: a batched job holds
+                // its members' graphs and their union, so it reserves twice their sum
+                uint64_t sum = j->reserve;
+                while (!s->queue.empty() && (int32_t)grp.size() < s->max_batch) {
+                    Job* k = s->queue.front();
+                    if (!batchable(s, k->req) || !same_run(k->req.run, j->req.run)) break;
+                    if (s->reserved + 2 * (sum + k->reserve) > s->budget) break;
+                    s->queue.pop_front();
+                    grp.push_back(k);
+                    sum += k->reserve;
+                }
+                if (grp.size() > 1) reserve = 2 * sum;
+            }
+            s->reserved += reserve;
+            const double t = s->now();
+            for (Job* k : grp) {
+                k->info.start_s = t;
+                k->info.slot = slot;
+                k->info.reserved_bytes = reserve;
+            }
         }
         s->cv_work.notify_all();  // the next request may fit too
-        run_job(s, slot, j);
+        if (grp.size() == 1) run_job(s, slot, grp[0]);
+        else run_group(s, slot, grp, reserve);
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->reserved -= j->reserve;
-            j->info.end_s = s->now();
-            j->done = true;
+            s->reserved -= reserve;
+            const double t = s->now();
+            for (Job* k : grp) {
+                k->info.end_s = t;
+                k->done = true;
+            }
         }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
@@ -154,6 +259,15 @@ int vt_server_create(int device, uint64_t hbm_budget_bytes, int32_t slots, vt_se
     }
     for (int i = 0; i < slots; i++) s->workers.emplace_back(worker, s, i);
     *out = s;
+    return VT_OK;
+}
+
+int vt_server_set_batching(vt_server* s, int32_t max_batch, int64_t max_n)
+{
+    if (!s || max_batch < 1 || max_batch > 4096 || max_n < 1) return VT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_batch = max_batch;
+    s->batch_max_n = max_n;
     return VT_OK;
 }
 
