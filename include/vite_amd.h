@@ -74,12 +74,17 @@ int vt_ctx_reset_stats(vt_ctx* ctx);
 typedef struct {
     int32_t bucket_max_degree[6];
     int32_t blocks_per_bucket;
+    /* Table size of the wide bucket (one wave per row, up to 1023 entries): rows up to
+     * [0] get 512 LDS slots, up to [1] 1024, the others 2048.  A table stays at most 3/4
+     * full, so [0] <= 383 and [1] <= 767 (-1: none); [0] <= [1]. */
+    int32_t wide_table_max_degree[2];
 } vt_tuning;
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out);
 int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t);
 /* Autotuner: from the state the first `prep` iterations of a phase-0 run (threshold 1e-3)
- * reach on g, times `reps` local-move sweeps under candidate bucket bounds and block caps
- * (coordinate descent from the current setting), keeps the fastest on ctx and returns it. */
+ * reach on g, times `reps` local-move sweeps under candidate bucket bounds, block caps and
+ * wide-row table sizes (coordinate descent from the current setting), keeps the fastest on
+ * ctx and returns it. */
 int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning* out);
 
 /* ---- graphs ------------------------------------------------------------------- */

@@ -230,9 +230,10 @@ def test_phase_sweep_state_replay(ctx, name, prep):
 
 
 TUNINGS = [dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=16384),
-           dict(bucket_max_degree=[5, 11, 23, 47, 95, 300], blocks_per_bucket=8),
-           dict(bucket_max_degree=[-1, -1, -1, -1, 127, 1023], blocks_per_bucket=64),
-           dict(bucket_max_degree=[7, 15, 31, 63, 127, 130], blocks_per_bucket=4096)]
+           dict(bucket_max_degree=[5, 11, 23, 47, 95, 300], blocks_per_bucket=8, wide_table_max_degree=[383, 767]),
+           dict(bucket_max_degree=[-1, -1, -1, -1, 127, 1023], blocks_per_bucket=64, wide_table_max_degree=[-1, -1]),
+           dict(bucket_max_degree=[7, 15, 31, 63, 127, 130], blocks_per_bucket=4096),
+           dict(bucket_max_degree=[-1, -1, -1, -1, -1, 1023], blocks_per_bucket=256, wide_table_max_degree=[150, 600])]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=[str(i) for i in range(len(TUNINGS))])
@@ -264,6 +265,9 @@ def test_tuning_rejects_bounds_beyond_kernel_capacity(ctx):
             ctx.set_tuning(bucket_max_degree=bad)
     with pytest.raises(vite_amd.VtError):
         ctx.set_tuning(blocks_per_bucket=4)
+    for bad in [[384, 511], [255, 768], [511, 300], [-2, 511]]:
+        with pytest.raises(vite_amd.VtError):
+            ctx.set_tuning(wide_table_max_degree=bad)
     assert ctx.tuning == default
 
 

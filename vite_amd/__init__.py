@@ -68,7 +68,13 @@ class _RunResult(C.Structure):
 
 # Every symbol include/vite_amd.h declares, with its ctypes signature.
 class _Tuning(C.Structure):
-    _fields_ = [("bucket_max_degree", C.c_int32 * 6), ("blocks_per_bucket", C.c_int32)]
+    _fields_ = [("bucket_max_degree", C.c_int32 * 6), ("blocks_per_bucket", C.c_int32),
+                ("wide_table_max_degree", C.c_int32 * 2)]
+
+
+def _tuning_dict(t) -> dict:
+    return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket,
+                wide_table_max_degree=list(t.wide_table_max_degree))
 
 
 class _Request(C.Structure):
@@ -219,18 +225,19 @@ class Context:
     def tuning(self) -> dict:
         t = _Tuning()
         self.check(_lib.vt_ctx_get_tuning(self.h, C.byref(t)))
-        return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket)
+        return _tuning_dict(t)
 
-    def set_tuning(self, bucket_max_degree=None, blocks_per_bucket=None):
+    def set_tuning(self, bucket_max_degree=None, blocks_per_bucket=None, wide_table_max_degree=None):
         cur = self.tuning
         t = _Tuning((C.c_int32 * 6)(*(bucket_max_degree or cur["bucket_max_degree"])),
-                    blocks_per_bucket or cur["blocks_per_bucket"])
+                    blocks_per_bucket or cur["blocks_per_bucket"],
+                    (C.c_int32 * 2)(*(wide_table_max_degree or cur["wide_table_max_degree"])))
         self.check(_lib.vt_ctx_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, g: "DeviceGraph", prep: int = 2, reps: int = 3) -> dict:
         t = _Tuning()
         self.check(_lib.vt_autotune(self.h, g.h, prep, reps, C.byref(t)))
-        return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket)
+        return _tuning_dict(t)
 
     def set_profiling(self, on: bool):
         self.check(_lib.vt_ctx_set_profiling(self.h, int(on)))
@@ -427,7 +434,7 @@ class Batch:
         """Picks the launch tuning for this batch's degree mix and keeps it on the context."""
         t = _Tuning()
         self.ctx.check(_lib.vt_batch_autotune(self.ctx.h, self.h, prep, reps, C.byref(t)))
-        return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket)
+        return _tuning_dict(t)
 
     def free(self):
         if self.h:

@@ -63,6 +63,7 @@ constexpr int kHubBucket = 6;
 // default is 7, 15, 31, 63, 127, 1023): rows above mx[5] take the HBM-table kernel.
 struct BucketBounds {
     int32_t mx[6];
+    int32_t wide[2];  // rows of bucket 5 up to wide[0] take the 512-slot table, up to wide[1] the 1024-slot one
 };
 
 __host__ __device__ inline int degree_bucket(int64_t d, const BucketBounds& bb)
@@ -1280,7 +1281,7 @@ __global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const 
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t d = offs[i + 1] - offs[i];
         uint32_t b = (uint32_t)(d <= lpv_max ? kLpvBucket : degree_bucket(d, bb));
-        if (b == kHubBucket - 1) b = d <= 255 ? kWide512Bucket : d <= 511 ? kWide1024Bucket : b;
+        if (b == kHubBucket - 1) b = d <= bb.wide[0] ? kWide512Bucket : d <= bb.wide[1] ? kWide1024Bucket : b;
         uint32_t c = 0;
         if (colors) c = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
         keys[i] = c * kNumBuckets + b;
@@ -1591,6 +1592,8 @@ struct Phase {
     {
         BucketBounds bb;
         for (int b = 0; b < 6; b++) bb.mx[b] = ctx->tuning.bucket_max_degree[b];
+        bb.wide[0] = ctx->tuning.wide_table_max_degree[0];
+        bb.wide[1] = ctx->tuning.wide_table_max_degree[1];
         return bb;
     }
 
@@ -2171,6 +2174,8 @@ vt_tuning default_tuning()
     vt_tuning t;
     for (int b = 0; b < 6; b++) t.bucket_max_degree[b] = kBucketCap[b];
     t.blocks_per_bucket = 16384;
+    t.wide_table_max_degree[0] = 255;  // tables at most half full
+    t.wide_table_max_degree[1] = 511;
     return t;
 }
 
@@ -2236,6 +2241,11 @@ static void check_tuning(vt_ctx* ctx, const vt_tuning* t)
                  cap[b]);
     if (t->blocks_per_bucket < 8 || t->blocks_per_bucket > (1 << 20))
         fail(ctx, VT_ERR_ARG, "vt_tuning: blocks_per_bucket = %d outside [8, 2^20]", t->blocks_per_bucket);
+    // a row has at most degree + 1 distinct communities; keep each table at most 3/4 full
+    const int32_t w0 = t->wide_table_max_degree[0], w1 = t->wide_table_max_degree[1];
+    if (w0 < -1 || w0 > 3 * 512 / 4 - 1 || w1 < -1 || w1 > 3 * 1024 / 4 - 1 || w0 > w1)
+        fail(ctx, VT_ERR_ARG, "vt_tuning: wide_table_max_degree = {%d, %d}: need -1 <= [0] <= [1], [0] <= 383, [1] <= 767",
+             w0, w1);
 }
 
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out)
@@ -2312,6 +2322,26 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
                 if (tt < 0.99 * tb) {
                     best = t;
                     tb = tt;
+                }
+            }
+            // wide-row table sizes: fuller small tables (more waves per CU) against emptier
+            // large ones (shorter probes), when the graph has wide rows at all
+            const ColorSweep& cs0 = st->ph.sweeps[0];
+            if (cs0.wide_n[0] + cs0.wide_n[1] + cs0.wide_n[2] > 0) {
+                for (int k = 0; k < 2; k++) {
+                    const std::vector<int> cands = k == 0 ? std::vector<int>{-1, 255, 319, 383}
+                                                          : std::vector<int>{511, 639, 767};
+                    for (int cand : cands) {
+                        if (cand == best.wide_table_max_degree[k]) continue;
+                        vt_tuning t = best;
+                        t.wide_table_max_degree[k] = cand;
+                        if (t.wide_table_max_degree[0] > t.wide_table_max_degree[1]) continue;
+                        const double tt = time_it(t);
+                        if (tt < 0.99 * tb) {
+                            best = t;
+                            tb = tt;
+                        }
+                    }
                 }
             }
             ctx->profiling = prof;
