@@ -1447,6 +1447,47 @@ __global__ void k_split_edges(const DEdge* __restrict__ edges, int64_t ne, int32
     }
 }
 
+// The edge split and the degree fold in one pass over the edges, for graphs of short rows: a
+// block takes kSfRows consecutive rows and walks their (contiguous) edges in tiles of
+// kSfTile with coalesced loads, writing the split copy and staging the weights in LDS;
+// thread t then adds row t's staged weights in order, carrying its sum across tiles -- the
+// same left fold as k_row_fold.
+constexpr int kSfRows = 64;
+constexpr int kSfTile = 2048;
+__global__ void __launch_bounds__(256) k_split_fold(const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
+                                                    int64_t nv, int32_t* __restrict__ tail32, double* __restrict__ wgt,
+                                                    double* __restrict__ vdeg)
+{
+    __shared__ double st[kSfTile];
+    for (int64_t r0 = (int64_t)blockIdx.x * kSfRows; r0 < nv; r0 += (int64_t)gridDim.x * kSfRows) {
+        const int nr = (int)(nv - r0 < kSfRows ? nv - r0 : kSfRows);
+        const int64_t b0 = offs[r0], b1 = offs[r0 + nr];
+        const int t = threadIdx.x;
+        int64_t e0 = 0, e1 = 0;
+        if (t < nr) {
+            e0 = offs[r0 + t];
+            e1 = offs[r0 + t + 1];
+        }
+        double acc = 0.0;
+        for (int64_t base = b0; base < b1; base += kSfTile) {
+            const int64_t top = base + kSfTile < b1 ? base + kSfTile : b1;
+            for (int64_t k = base + t; k < top; k += 256) {
+                const DEdge e = edges[k];
+                tail32[k] = (int32_t)e.tail;
+                wgt[k] = e.weight;
+                st[k - base] = e.weight;
+            }
+            __syncthreads();
+            if (t < nr) {
+                const int64_t lo = e0 > base ? e0 : base, hi = e1 < top ? e1 : top;
+                for (int64_t k = lo; k < hi; k++) acc = acc + st[k - base];
+            }
+            __syncthreads();
+        }
+        if (t < nr) vdeg[r0 + t] = acc;
+    }
+}
+
 __global__ void k_narrow_ids(const int64_t* __restrict__ src, int64_t n, int32_t* __restrict__ dst)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1625,7 +1666,15 @@ struct Phase {
 
     void init_state(bool with_et)
     {
-        if (nv && nv < 65536 && g->ne > 48 * nv)  // few, long rows: a thread per row would leave the GPU idle
+        if (kSplitEdges && !tail32.p && nv && g->ne <= 48 * nv) {
+            // short rows: the split copy the local move reads and the degrees in one pass
+            const size_t ne1 = (size_t)std::max<int64_t>(g->ne, 1);
+            tail32 = DBuf<int32_t>(ctx, ne1);
+            wgt = DBuf<double>(ctx, ne1);
+            curr32 = DBuf<int32_t>(ctx, (size_t)nv);
+            VT_LAUNCH(ctx, "split_fold", k_split_fold, grid_for(nv, kSfRows, 65536), 256, 0, g->offs, g->edges, nv,
+                      tail32.p, wgt.p, vdeg.p);
+        } else if (nv && nv < 65536 && g->ne > 48 * nv)  // few, long rows: a thread per row would leave the GPU idle
             VT_LAUNCH(ctx, "row_fold", k_row_fold_wave, grid_for(nv, kFoldWaveRows, 65536), 64 * kFoldWaveRows, 0, g->offs,
                       g->edges, nv, vdeg.p);
         else if (nv)
