@@ -1349,9 +1349,11 @@ __global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, cons
 // event's community, -k_v when it leaves (a - x == a + (-x) exactly), and +-1 to the member
 // count.  The per-community folds below then read them as a stream.
 __global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                               const int64_t* __restrict__ target, const double* __restrict__ vdeg, uint32_t none,
-                               double* __restrict__ delta, int8_t* __restrict__ dir)
+                               const int64_t* __restrict__ d_movers, const int64_t* __restrict__ target,
+                               const double* __restrict__ vdeg, uint32_t none, double* __restrict__ delta,
+                               int8_t* __restrict__ dir)
 {
+    if (d_movers) n = 2 * d_movers[0];  // the count the device computed (a replayed graph's n is fixed)
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
         if (c == none) continue;
@@ -1368,9 +1370,10 @@ __global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t
 // updates does (+= k_i on arrival, -= k_i on departure), then localCinfo += localCupdate
 // (louvain.cpp:2534-2547).
 __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* __restrict__ delta,
-                              const int8_t* __restrict__ dir, int64_t n, double* __restrict__ cdeg,
-                              int64_t* __restrict__ csize, uint32_t none)
+                              const int8_t* __restrict__ dir, int64_t n, const int64_t* __restrict__ d_movers,
+                              double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none)
 {
+    if (d_movers) n = 2 * d_movers[0];
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
         if (c == none || (j > 0 && keys[j - 1] == c)) continue;
@@ -1506,17 +1509,13 @@ __global__ void k_seg_constant(const double* __restrict__ sums, int32_t ng, doub
 // vertex that does not move, or whose graph has left the phase, gets key `none`, which sorts
 // after every community.  A stable sort then gives each community its events in vertex
 // order, as the one-graph path does with a compacted mover list.
-__global__ void k_batch_events(int64_t nv, const int64_t* __restrict__ target, const int64_t* __restrict__ curr,
-                               const int32_t* __restrict__ gid, const unsigned char* __restrict__ gact, uint32_t none,
-                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// A batch vertex moves when its graph is still iterating and its target differs.
+__global__ void k_batch_flags(int64_t nv, const int64_t* __restrict__ target, const int64_t* __restrict__ curr,
+                              const int32_t* __restrict__ gid, const unsigned char* __restrict__ gact,
+                              int32_t* __restrict__ flags)
 {
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
-        const bool mv = gact[gid[v]] && target[v] != curr[v];
-        keys[2 * v] = mv ? (uint32_t)target[v] : none;
-        vals[2 * v] = (uint32_t)v;
-        keys[2 * v + 1] = mv ? (uint32_t)curr[v] : none;
-        vals[2 * v + 1] = (uint32_t)v;
-    }
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x)
+        flags[v] = (gact[gid[v]] && target[v] != curr[v]) ? 1 : 0;
 }
 
 // The end of the reference loop body per graph (louvain.cpp:540-577): currMod from the
@@ -1925,13 +1924,15 @@ struct Phase {
     }
 
     // localCinfo += the sorted events' folds, per community in processing order.
+    // ne: the event count, or its upper bound when d_movers (the mover count on the device)
+    // is given.
     void fold_events(const uint32_t* ks, const uint32_t* vs, int64_t ne, uint32_t none, double* delta, int8_t* dir,
-                     double* dst_cdeg, int64_t* dst_csize)
+                     double* dst_cdeg, int64_t* dst_csize, const int64_t* d_movers = nullptr)
     {
-        VT_LAUNCH(ctx, "event_deltas", k_event_deltas, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne,
+        VT_LAUNCH(ctx, "event_deltas", k_event_deltas, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, d_movers,
                   (const int64_t*)target.p, (const double*)vdeg.p, none, delta, dir);
         VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, (const double*)delta,
-                  (const int8_t*)dir, ne, dst_cdeg, dst_csize, none);
+                  (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none);
     }
 
     double modularity(int flags)
@@ -2137,18 +2138,42 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     uint32_t* v1 = (uint32_t*)scratch(ctx, "bev_v1", 4 * (size_t)nev);
     double* dl = (double*)scratch(ctx, "bev_d", 8 * (size_t)nev);
     int8_t* dr = (int8_t*)scratch(ctx, "bev_s", (size_t)nev);
+    int32_t* mflags = (int32_t*)scratch(ctx, "bmv_flags", 4 * (size_t)std::max<int64_t>(nv, 1));
+    int32_t* mpos = (int32_t*)scratch(ctx, "bmv_pos", 4 * (size_t)std::max<int64_t>(nv, 1));
+    DBuf<int64_t> dmov(ctx, 1);
     const uint32_t none = (uint32_t)nv;
     const int bits = (int)bits_for(std::max<int64_t>(nv, 1));
-    bool alt = false;
+    // which buffer pair the folds read the sorted events from: fixed once the folds are
+    // captured into a graph, and a sort that ends in the other pair is copied over
+    bool exp_alt = false, exp_fixed = false;
 
     auto pre = [&]() {
         ph.sweep(ph.sweeps[0]);
-        VT_LAUNCH(ctx, "batch_events", k_batch_events, grid_for(nv, 256, 8192), 256, 0, nv, (const int64_t*)ph.target.p,
-                  (const int64_t*)ph.curr.p, d_gid, (const unsigned char*)gact.p, none, k0, v0);
+        VT_LAUNCH(ctx, "batch_flags", k_batch_flags, grid_for(nv, 256, 8192), 256, 0, nv, (const int64_t*)ph.target.p,
+                  (const int64_t*)ph.curr.p, d_gid, (const unsigned char*)gact.p, mflags);
     };
-    auto sort = [&]() { sort_pairs_u32(ctx, k0, v0, 2 * nv, bits, k1, v1, &alt); };
+    // between the graphs: the movers' events compacted and sorted (the library scan and
+    // sort stay outside capture; the sort's size is the mover count, read back here)
+    auto sort = [&]() {
+        exclusive_scan_i32(ctx, mflags, mpos, nv);
+        VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)mflags, (const int32_t*)mpos, nv, dmov.p);
+        int64_t movers = 0;
+        d2h(ctx, &movers, dmov.p, 8);
+        if (movers == 0) return;
+        VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nv, 256, 8192), 256, 0, (const int32_t*)nullptr, nv,
+                  (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p, (const int64_t*)ph.curr.p,
+                  k0, v0);
+        bool alt = false;
+        sort_pairs_u32(ctx, k0, v0, 2 * movers, bits, k1, v1, &alt);
+        if (!exp_fixed) {
+            exp_alt = alt;
+        } else if (alt != exp_alt) {
+            d2d(ctx, exp_alt ? k1 : k0, alt ? k1 : k0, 4 * 2 * (size_t)movers);
+            d2d(ctx, exp_alt ? v1 : v0, alt ? v1 : v0, 4 * 2 * (size_t)movers);
+        }
+    };
     auto post = [&]() {
-        ph.fold_events(alt ? k1 : k0, alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p);
+        ph.fold_events(exp_alt ? k1 : k0, exp_alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p, dmov.p);
         if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
         else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
         dzero(ctx, nact.p, 4);
@@ -2191,6 +2216,7 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
             if (use_graph && rounds == 1) {
                 capture(pre, &gpre);
                 capture(post, &gpost);
+                exp_fixed = true;
             }
             if (gpre) {
                 VT_HIP(ctx, hipGraphLaunch(gpre, ctx->stream));
