@@ -2153,13 +2153,18 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
                   (const int64_t*)ph.curr.p, d_gid, (const unsigned char*)gact.p, mflags);
     };
     // between the graphs: the movers' events compacted and sorted (the library scan and
-    // sort stay outside capture; the sort's size is the mover count, read back here)
-    auto sort = [&]() {
+    // sort stay outside capture; the sort's size is the mover count, read back here, and in
+    // the same wait the count of graphs the last round left iterating).  Returns false when
+    // none was left: the sweep just run skipped every vertex, and the phase is over.
+    int32_t active = ng;
+    auto sort = [&]() -> bool {
         exclusive_scan_i32(ctx, mflags, mpos, nv);
         VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)mflags, (const int32_t*)mpos, nv, dmov.p);
         int64_t movers = 0;
         d2h(ctx, &movers, dmov.p, 8);
-        if (movers == 0) return;
+        d2h(ctx, &active, nact.p, 4);
+        if (active == 0) return false;
+        if (movers == 0) return true;
         VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nv, 256, 8192), 256, 0, (const int32_t*)nullptr, nv,
                   (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p, (const int64_t*)ph.curr.p,
                   k0, v0);
@@ -2171,6 +2176,7 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
             d2d(ctx, exp_alt ? k1 : k0, alt ? k1 : k0, 4 * 2 * (size_t)movers);
             d2d(ctx, exp_alt ? v1 : v0, alt ? v1 : v0, 4 * 2 * (size_t)movers);
         }
+        return true;
     };
     auto post = [&]() {
         ph.fold_events(exp_alt ? k1 : k0, exp_alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p, dmov.p);
@@ -2185,7 +2191,6 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     };
 
     int rounds = 0;
-    int32_t active = ng;
     hipGraphExec_t gpre = nullptr, gpost = nullptr;
     auto capture = [&](auto&& body, hipGraphExec_t* exec) {
         const bool prof = ctx->profiling;
@@ -2211,24 +2216,20 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
         if (gpost) (void)hipGraphExecDestroy(gpost);
         gpre = gpost = nullptr;
     };
+    h2d(ctx, nact.p, &active, 4);  // every graph iterates in the first round
     try {
-        while (active > 0) {
-            if (use_graph && rounds == 1) {
+        for (;;) {
+            if (use_graph && rounds == 1 && !gpre) {
                 capture(pre, &gpre);
                 capture(post, &gpost);
                 exp_fixed = true;
             }
-            if (gpre) {
-                VT_HIP(ctx, hipGraphLaunch(gpre, ctx->stream));
-                sort();
-                VT_HIP(ctx, hipGraphLaunch(gpost, ctx->stream));
-            } else {
-                pre();
-                sort();
-                post();
-            }
+            if (gpre) VT_HIP(ctx, hipGraphLaunch(gpre, ctx->stream));
+            else pre();
+            if (!sort()) break;
+            if (gpost) VT_HIP(ctx, hipGraphLaunch(gpost, ctx->stream));
+            else post();
             rounds++;
-            d2h(ctx, &active, nact.p, 4);
         }
     } catch (...) {
         destroy();
