@@ -1274,8 +1274,11 @@ __global__ void k_constant(const double* __restrict__ sums, double* __restrict__
 }
 
 // Sort keys for the sweep lists: (color bucket, degree bucket).
+// In a batch (gid), a vertex whose graph has left the phase gets key `excluded`, past every
+// list, so the sweeps no longer visit it.
 __global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const int64_t* __restrict__ colors,
-                            int64_t num_color, int lpv_max, BucketBounds bb, uint32_t* __restrict__ keys,
+                            int64_t num_color, int lpv_max, BucketBounds bb, const int32_t* __restrict__ gid,
+                            const unsigned char* __restrict__ gact, uint32_t excluded, uint32_t* __restrict__ keys,
                             uint32_t* __restrict__ vals)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1284,7 +1287,7 @@ __global__ void k_list_keys(const int64_t* __restrict__ offs, int64_t nv, const 
         if (b == kHubBucket - 1) b = d <= bb.wide[0] ? kWide512Bucket : d <= bb.wide[1] ? kWide1024Bucket : b;
         uint32_t c = 0;
         if (colors) c = (uint32_t)(colors[i] < 0 ? num_color - 1 : colors[i]);
-        keys[i] = c * kNumBuckets + b;
+        keys[i] = (gid && !gact[gid[i]]) ? excluded : c * kNumBuckets + b;
         vals[i] = (uint32_t)i;
     }
 }
@@ -1305,7 +1308,7 @@ __global__ void k_key_histogram(const uint32_t* __restrict__ keys, int64_t n, in
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_bins[i] = 0;
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&s_bins[keys[i]], 1u);
+        if (keys[i] < (uint32_t)nbins) atomicAdd(&s_bins[keys[i]], 1u);  // larger keys: excluded
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
         if (s_bins[i]) atomicAdd((unsigned long long*)&hist[i], (unsigned long long)s_bins[i]);
@@ -1707,8 +1710,8 @@ struct Phase {
         std::vector<int64_t> h(nkeys, 0);
         if (nv) {
             VT_LAUNCH(ctx, "list_keys", k_list_keys, grid_for(nv, 256, 8192), 256, 0, g->offs, nv,
-                      ncol > 1 ? d_colors : nullptr, num_color, gid ? 0 : lpv_max_degree(), bounds(), k0,
-                      (uint32_t*)lists.p);
+                      ncol > 1 ? d_colors : nullptr, num_color, gid ? 0 : lpv_max_degree(), bounds(), gid, gact,
+                      (uint32_t)nkeys, k0, (uint32_t*)lists.p);
             dzero(ctx, hist, 8 * nkeys);
             key_histogram(k0, nv, hist, (int)nkeys);
             bool alt = false;
@@ -2156,7 +2159,12 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     // sort stay outside capture; the sort's size is the mover count, read back here, and in
     // the same wait the count of graphs the last round left iterating).  Returns false when
     // none was left: the sweep just run skipped every vertex, and the phase is over.
-    int32_t active = ng;
+    int32_t active = ng, last_active = ng;
+    // the sweep lists drop the vertices of finished graphs once those are half the listed ones
+    std::vector<int64_t> hvoff((size_t)ng + 1);
+    d2h(ctx, hvoff.data(), seg.voff.p, 8 * ((size_t)ng + 1));
+    int64_t listed = nv;
+    bool relist = false;
     auto sort = [&]() -> bool {
         exclusive_scan_i32(ctx, mflags, mpos, nv);
         VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)mflags, (const int32_t*)mpos, nv, dmov.p);
@@ -2164,6 +2172,18 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
         d2h(ctx, &movers, dmov.p, 8);
         d2h(ctx, &active, nact.p, 4);
         if (active == 0) return false;
+        if (active < last_active) {
+            last_active = active;
+            std::vector<unsigned char> ga((size_t)ng);
+            d2h(ctx, ga.data(), gact.p, (size_t)ng);
+            int64_t live = 0;
+            for (int32_t k = 0; k < ng; k++)
+                if (ga[k]) live += hvoff[k + 1] - hvoff[k];
+            if (2 * live < listed) {
+                relist = true;
+                listed = live;
+            }
+        }
         if (movers == 0) return true;
         VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nv, 256, 8192), 256, 0, (const int32_t*)nullptr, nv,
                   (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p, (const int64_t*)ph.curr.p,
@@ -2219,8 +2239,8 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     h2d(ctx, nact.p, &active, 4);  // every graph iterates in the first round
     try {
         for (;;) {
-            if (use_graph && rounds == 1 && !gpre) {
-                capture(pre, &gpre);
+            if (use_graph && rounds >= 1 && !gpre) capture(pre, &gpre);
+            if (use_graph && rounds >= 1 && !gpost) {
                 capture(post, &gpost);
                 exp_fixed = true;
             }
@@ -2230,6 +2250,13 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
             if (gpost) VT_HIP(ctx, hipGraphLaunch(gpost, ctx->stream));
             else post();
             rounds++;
+            if (relist) {
+                // new lists (after this round's decisions): the captured sweep is re-captured
+                relist = false;
+                ph.build_lists(nullptr, 1, false);
+                if (gpre) (void)hipGraphExecDestroy(gpre);
+                gpre = nullptr;
+            }
         }
     } catch (...) {
         destroy();
