@@ -213,8 +213,11 @@ def skewed_form(ctx, args):
     ctx.set_tuning(**saved)
     log(f"skewed: sweep {1e3 * wall:.3f} ms ({dg.ne / wall / 1e9:.2f} G edges/s) {kernels}; "
         f"-c 8 -t 2 -a 0.25 run {run_s:.2f}s Q={r.final_q:.9f}")
+    # the same algorithmic bytes as the headline (24E + 56V) over the whole sweep's wall time
+    sweep_bytes = 24.0 * float(deg.sum()) + 56.0 * float(len(deg))
     return {"n": int(len(deg)), "directed_entries": int(deg.sum()), "max_degree": int(deg.max()),
-            "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall, "kernel_avg_ms": kernels,
+            "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall,
+            "sweep_hbm_frac": sweep_bytes / wall / 1e9 / HBM_PEAK_GBS, "kernel_avg_ms": kernels,
             "run_seconds": run_s, "final_q": r.final_q, "levels": len(r.level_q), "colors": r.num_colors,
             "flags": "-c 8 -t 2 -a 0.25", "tuning": tuning}
 

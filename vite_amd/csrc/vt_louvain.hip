@@ -1668,8 +1668,9 @@ struct Phase {
 
     void init_state(bool with_et)
     {
-        if (kSplitEdges && !tail32.p && nv && g->ne <= 48 * nv) {
-            // short rows: the split copy the local move reads and the degrees in one pass
+        if (kSplitEdges && !tail32.p && nv && g->ne <= 48 * nv && g->max_degree <= kSfTile) {
+            // short rows (a row's fold is one thread's, so no long ones): the split copy the
+            // local move reads and the degrees in one pass
             const size_t ne1 = (size_t)std::max<int64_t>(g->ne, 1);
             tail32 = DBuf<int32_t>(ctx, ne1);
             wgt = DBuf<double>(ctx, ne1);
