@@ -233,6 +233,7 @@ typedef struct {
     int32_t phases;   /* batched phases run (rounds of the phase loop)             */
     int32_t rounds;   /* batched iterations over all phases                        */
     double seconds;   /* wall time of the whole batched run                        */
+    int32_t relists;  /* times the sweep lists dropped the vertices of finished graphs */
 } vt_batch_stats;
 /* Copies n device graphs into a batch; the graphs stay the caller's.  Each needs at least one
  * edge (with 2m = 0 modularity is undefined and the reference's loop never ends). */

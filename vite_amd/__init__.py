@@ -89,7 +89,7 @@ class _RequestInfo(C.Structure):
 
 
 class _BatchStats(C.Structure):
-    _fields_ = [("phases", C.c_int32), ("rounds", C.c_int32), ("seconds", C.c_double)]
+    _fields_ = [("phases", C.c_int32), ("rounds", C.c_int32), ("seconds", C.c_double), ("relists", C.c_int32)]
 
 
 _SIGS = {
@@ -428,7 +428,8 @@ class Batch:
         res = (_RunResult * self.n)()
         st = _BatchStats()
         self.ctx.check(_lib.vt_batch_run(self.ctx.h, self.h, C.byref(rp), res, C.byref(st)))
-        return [_run_result(r) for r in res], dict(phases=st.phases, rounds=st.rounds, seconds=st.seconds)
+        return [_run_result(r) for r in res], dict(phases=st.phases, rounds=st.rounds, seconds=st.seconds,
+                                                   relists=st.relists)
 
     def autotune(self, prep: int = 1, reps: int = 2) -> dict:
         """Picks the launch tuning for this batch's degree mix and keeps it on the context."""

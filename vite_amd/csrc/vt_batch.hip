@@ -175,7 +175,7 @@ void record_call(vt_run_result& r, double q, int32_t iters)
 // round.  A graph whose loop would run its extra 1e-6 pass (threshold cycling, early exit)
 // takes part in the next round with its current graph instead of the coarsened one.
 void batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* rp, vt_run_result* res, int* phases_out,
-               int* rounds_out)
+               int* rounds_out, int* relists_out)
 {
     if (rp->coloring || rp->vertex_order || rp->et_type != 0)
         fail(ctx, VT_ERR_LIMIT, "vt_batch_run: the batched form runs plain Louvain (no coloring, no early termination)");
@@ -195,7 +195,7 @@ void batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* rp, vt_run_result*
     DBuf<int32_t> cur_gid_owned;
     const int32_t* cur_gid = b->gid.p;
     std::vector<int64_t> voff = b->voff, eoff = b->eoff;
-    int phases = 0, rounds_total = 0;
+    int phases = 0, rounds_total = 0, relists_total = 0;
     auto drop_cur = [&]() {
         if (cur_owned) graph_delete(ctx, cur);
         else if (cur->part) {
@@ -234,7 +234,9 @@ void batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* rp, vt_run_result*
             std::vector<double> q;
             std::vector<int32_t> it;
             int rounds = 0;
-            batch_phase(ctx, cur, cur_gid, seg, lower, thresh, sum_flags, use_graph, q, it, &rounds);
+            int relists = 0;
+            batch_phase(ctx, cur, cur_gid, seg, lower, thresh, sum_flags, use_graph, q, it, &rounds, &relists);
+            relists_total += relists;
             phases++;
             rounds_total += rounds;
 
@@ -360,6 +362,7 @@ void batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* rp, vt_run_result*
     }
     if (phases_out) *phases_out = phases;
     if (rounds_out) *rounds_out = rounds_total;
+    if (relists_out) *relists_out = relists_total;
 }
 
 }  // namespace
@@ -413,13 +416,14 @@ int vt_batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* p, vt_run_result
     return guarded(ctx, [&] {
         VT_HIP(ctx, hipSetDevice(ctx->device));
         const double t0 = now_s();
-        int phases = 0, rounds = 0;
-        batch_run(ctx, b, p, results, &phases, &rounds);
+        int phases = 0, rounds = 0, relists = 0;
+        batch_run(ctx, b, p, results, &phases, &rounds, &relists);
         sync(ctx);
         if (stats) {
             stats->phases = phases;
             stats->rounds = rounds;
             stats->seconds = now_s() - t0;
+            stats->relists = relists;
         }
     });
 }

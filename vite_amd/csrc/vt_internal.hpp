@@ -225,7 +225,7 @@ vt_tuning default_tuning();  // bucket bounds at each kernel's capacity (vt_louv
 // ---- batch of disjoint graphs (vt_louvain.hip) ----------------------------------------------
 void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,
                  const std::vector<double>& thresh, int flags, bool use_graph, std::vector<double>& q_out,
-                 std::vector<int32_t>& iters_out, int* rounds_out);
+                 std::vector<int32_t>& iters_out, int* rounds_out, int* relists_out = nullptr);
 vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_part);
 
 }  // namespace vt

@@ -2106,7 +2106,7 @@ double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cv
 // stays outside capture because the library sort clears its buffers with memsets.
 void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,
                  const std::vector<double>& thresh, int flags, bool use_graph, std::vector<double>& q_out,
-                 std::vector<int32_t>& iters_out, int* rounds_out)
+                 std::vector<int32_t>& iters_out, int* rounds_out, int* relists_out)
 {
     const int32_t ng = seg.ng;
     const int64_t nv = u->nv;
@@ -2166,6 +2166,7 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     d2h(ctx, hvoff.data(), seg.voff.p, 8 * ((size_t)ng + 1));
     int64_t listed = nv;
     bool relist = false;
+    int relists = 0;
     auto sort = [&]() -> bool {
         exclusive_scan_i32(ctx, mflags, mpos, nv);
         VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)mflags, (const int32_t*)mpos, nv, dmov.p);
@@ -2254,6 +2255,7 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
             if (relist) {
                 // new lists (after this round's decisions): the captured sweep is re-captured
                 relist = false;
+                relists++;
                 ph.build_lists(nullptr, 1, false);
                 if (gpre) (void)hipGraphExecDestroy(gpre);
                 gpre = nullptr;
@@ -2271,6 +2273,7 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     d2h(ctx, q_out.data(), gprev.p, 8 * (size_t)ng);
     d2h(ctx, iters_out.data(), giters.p, 4 * (size_t)ng);
     if (rounds_out) *rounds_out = rounds;
+    if (relists_out) *relists_out = relists;
 }
 
 vt_tuning default_tuning()
