@@ -29,9 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Rows of degree <= 63 run in the fused local-move kernel; degree classes above it have
-# their own launches (mid, wide, hub).
-FUSED_MAX_DEGREE = 63
+# Rows of bucket 2 (16-lane groups; degree <= 47 with the default bounds) run in the
+# launch named "local_move", the dominant one on random geometric graphs; buckets 0, 1
+# and 3 run in "local_move_rest" and larger degree classes in their own launches.
+G16_BUCKET = 2
+G16_KERNEL = "k_local_move<false, 2>"
 # rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) against the bytes a plain 1 GiB
 # device copy really moves: FETCH_SIZE reads half of them, WRITE_SIZE all of them.
 FETCH_SCALE = 2.0
@@ -200,7 +202,7 @@ def skewed_form(ctx, args):
         st.sweep()
     wall = (time.perf_counter() - t0) / reps
     kernels = {}
-    for k in ("local_move", "local_move_mid", "local_move_wide", "local_move_hub"):
+    for k in ("local_move", "local_move_rest", "local_move_mid", "local_move_wide", "local_move_hub"):
         avg, cnt, _ = ctx.kernel_stats(k)
         if cnt:
             kernels[k] = avg
@@ -286,17 +288,28 @@ def serving_form(args):
             "workload": "Poisson arrivals; RGG or BA (m=8) 50/50, n log-uniform in [2^10, 2^20]; threshold cycling"}
 
 
-def measured_traffic(kernel, workload_tag):
-    """HBM bytes per launch of `kernel` from the committed rocprof counter summary of this
-    workload (FETCH_SIZE x FETCH_SCALE + WRITE_SIZE x WRITE_SCALE), or None."""
+def degree_buckets(deg, bucket_max_degree):
+    """Bucket of each row as the sweep assigns it: the first bucket b with degree <=
+    bucket_max_degree[b] (-1 leaves b empty), else 6 (mid and wider)."""
+    import numpy as np
+    b = np.full(len(deg), 6, dtype=np.int8)
+    for i in range(5, -1, -1):
+        if bucket_max_degree[i] >= 0:
+            b[deg <= bucket_max_degree[i]] = i
+    return b
+
+
+def measured_traffic(kernel, workload_tag, symbol):
+    """HBM bytes per launch of the kernel `symbol` (as rocprofv3 names it, without "vt::")
+    from the committed rocprof counter summary of launch `kernel` on this workload
+    (FETCH_SIZE x FETCH_SCALE + WRITE_SIZE x WRITE_SCALE), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}_fetch_write.txt")
     if not os.path.exists(path):
         return None, None
     vals = {}
     for line in open(path):
         parts = [p.strip() for p in line.split("|")]
-        name = re.sub(r"<.*>", "", parts[0]) if parts else ""
-        if len(parts) == 4 and name.endswith("k_" + kernel) and parts[1] in ("FETCH_SIZE", "WRITE_SIZE"):
+        if len(parts) == 4 and parts[0].endswith("vt::" + symbol) and parts[1] in ("FETCH_SIZE", "WRITE_SIZE"):
             vals[parts[1]] = float(parts[3])
     if len(vals) != 2:
         return None, None
@@ -382,9 +395,9 @@ def main():
     state = st.state()
     st.end(want_cvect=False)
 
-    # the dominant kernel's own share: rows of degree <= FUSED_MAX_DEGREE
+    # the dominant kernel's own share: the rows of bucket 2 under the tuning in force
     deg = np.diff(dg.offsets())
-    fused = deg <= FUSED_MAX_DEGREE
+    fused = degree_buckets(deg, tuning["bucket_max_degree"]) == G16_BUCKET
     Vf, Ef = int(np.count_nonzero(fused)), int(deg[fused].sum())
     alg_bytes = 24 * Ef + 56 * Vf
     achieved = alg_bytes / (kavg * 1e-3) / 1e9 if kavg > 0 else 0.0
@@ -393,7 +406,7 @@ def main():
         f"({Vf} of {V} rows, {Ef} of {E} entries), {achieved:.1f} GB/s algorithmic "
         f"({100 * achieved / HBM_PEAK_GBS:.1f}% of peak), {value / 1e9:.2f} G edges/s")
     tag = f"rgg_n{V}"
-    traffic, traffic_src = measured_traffic("local_move", tag)
+    traffic, traffic_src = measured_traffic("local_move", tag, G16_KERNEL)
 
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -405,8 +418,10 @@ def main():
                    "threshold": 1e-3, "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "local_move", "kernel_avg_ms": kavg, "algorithmic_bytes_per_launch": alg_bytes,
-                     "bytes_formula": "24E + 56V over the kernel's rows (degree <= 63)",
+                     "kernel": G16_KERNEL, "kernel_avg_ms": kavg, "algorithmic_bytes_per_launch": alg_bytes,
+                     "bytes_formula": "24E + 56V over the kernel's rows (bucket 2: 16-lane groups)",
+                     "sweep_achieved": (24 * E + 56 * V) / (ms_per_step * 1e-3) / 1e9,
+                     "sweep_formula": "24E + 56V over all rows / wall time of the whole sweep (every launch)",
                      "kernel_rows": Vf, "kernel_entries": Ef,
                      "traffic_source": traffic_src,
                      "traffic_formula": f"FETCH_SIZE x {FETCH_SCALE} + WRITE_SIZE x {WRITE_SCALE} (KiB per dispatch)"},

@@ -782,9 +782,12 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
     }
 }
 
-// Buckets 0-3 (degree <= 63) of one sweep in a single launch: blocks [bbeg[b], bbeg[b+1])
-// run bucket b over its list range.  Bucket 4 (64 <= degree <= 127) has its own launch so
-// that its larger tables do not set the LDS footprint, and so the occupancy, of this one.
+// Buckets 0-3 (degree <= 63) of one sweep in two launches over one block plan: blocks
+// [bbeg[b], bbeg[b+1]) run bucket b over its list range.  Bucket 2 (16-lane groups) runs
+// alone and buckets 0, 1 and 3 share the other launch: each kernel then fits 64 VGPRs,
+// so a SIMD holds eight waves (one kernel for all four needs 70 and holds seven).
+// Bucket 4 (64 <= degree <= 127) has its own launch so that its larger tables do not set
+// the LDS footprint, and so the occupancy, of these.
 constexpr int kFusedBuckets = 4;
 constexpr int kFusedBlock = 256;
 constexpr size_t kFusedLds = std::max({GroupLds<4, 16, 256, kNch[0]>::bytes, GroupLds<8, VT_CAP_G8, 256, kNch[1]>::bytes,
@@ -798,25 +801,31 @@ struct SweepPlan {
     int64_t n[kFusedBuckets];         // list lengths
 };
 
-#ifndef VT_FUSED_WPE
-#define VT_FUSED_WPE 7
-#endif
-template <bool BATCH>
-__global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(VT_FUSED_WPE)))
+enum FusedPart { kPartRest = 1, kPartG16 = 2 };
+
+template <bool BATCH, int PART>
+__global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
 {
     __shared__ __align__(16) char lds[kFusedLds];
     __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
+    // kPartRest skips bucket 2's block range; kPartG16 covers only that range
+    int64_t bid = blockIdx.x;
+    if (PART == kPartRest && bid >= plan.bbeg[2]) bid += plan.bbeg[3] - plan.bbeg[2];
+    if (PART == kPartG16) bid += plan.bbeg[2];
     int b = 0;
-    while (b < kFusedBuckets - 1 && (int64_t)blockIdx.x >= plan.bbeg[b + 1]) b++;
-    const int64_t blk = blockIdx.x - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
+    while (b < kFusedBuckets - 1 && bid >= plan.bbeg[b + 1]) b++;
+    const int64_t blk = bid - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
     const int32_t* list = lists + plan.off[b];
-    switch (b) {
-        case 0: run_bucket<4, 16, 256, true, BATCH, kNch[0]>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
-        case 1: run_bucket<8, VT_CAP_G8, 256, true, BATCH, kNch[1]>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
-        case 2: run_bucket<16, 64, 256, true, BATCH, kNch[2]>(a, lds, list, plan.n[2], blk, nblk, &s_inactive); break;
-        default: run_bucket<32, 128, 256, true, BATCH, kNch[3]>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+    if (PART == kPartG16) {
+        run_bucket<16, 64, 256, true, BATCH, kNch[2]>(a, lds, list, plan.n[2], blk, nblk, &s_inactive);
+    } else {
+        switch (b) {
+            case 0: run_bucket<4, 16, 256, true, BATCH, kNch[0]>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
+            case 1: run_bucket<8, VT_CAP_G8, 256, true, BATCH, kNch[1]>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+            default: run_bucket<32, 128, 256, true, BATCH, kNch[3]>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
@@ -1850,9 +1859,16 @@ struct Phase {
         }
         const bool batch = a.gid != nullptr;
         if (cs.fused_blocks) {
-            auto kern = batch ? k_local_move<true> : k_local_move<false>;
-            VT_LAUNCH(ctx, "local_move", kern, (unsigned)cs.fused_blocks, kFusedBlock, 0, a, (const int32_t*)lists.p,
-                      cs.plan);
+            const int64_t n16 = cs.plan.bbeg[3] - cs.plan.bbeg[2];
+            if (cs.fused_blocks > n16) {
+                auto kern = batch ? k_local_move<true, kPartRest> : k_local_move<false, kPartRest>;
+                VT_LAUNCH(ctx, "local_move_rest", kern, (unsigned)(cs.fused_blocks - n16), kFusedBlock, 0, a,
+                          (const int32_t*)lists.p, cs.plan);
+            }
+            if (n16) {
+                auto kern = batch ? k_local_move<true, kPartG16> : k_local_move<false, kPartG16>;
+                VT_LAUNCH(ctx, "local_move", kern, (unsigned)n16, kFusedBlock, 0, a, (const int32_t*)lists.p, cs.plan);
+            }
         }
         if (cs.mid_n) {
             auto kern = batch ? k_local_move_mid<true> : k_local_move_mid<false>;
