@@ -233,7 +233,10 @@ TUNINGS = [dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=16
            dict(bucket_max_degree=[5, 11, 23, 47, 95, 300], blocks_per_bucket=8, wide_table_max_degree=[383, 767]),
            dict(bucket_max_degree=[-1, -1, -1, -1, 127, 1023], blocks_per_bucket=64, wide_table_max_degree=[-1, -1]),
            dict(bucket_max_degree=[7, 15, 31, 63, 127, 130], blocks_per_bucket=4096),
-           dict(bucket_max_degree=[-1, -1, -1, -1, -1, 1023], blocks_per_bucket=256, wide_table_max_degree=[150, 600])]
+           dict(bucket_max_degree=[-1, -1, -1, -1, -1, 1023], blocks_per_bucket=256, wide_table_max_degree=[150, 600]),
+           # bucket 2 (16-lane groups, a launch of its own) alone, then everything but it
+           dict(bucket_max_degree=[-1, -1, 47, -1, 127, 1023], blocks_per_bucket=512),
+           dict(bucket_max_degree=[7, 15, -1, 63, 127, 1023], blocks_per_bucket=512)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=[str(i) for i in range(len(TUNINGS))])
