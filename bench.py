@@ -82,6 +82,7 @@ def summarize_profile(d):
     import glob
 
     def short(name):
+        name = name.replace("(anonymous namespace)::", "")
         name = name.split("(")[0] if not name.startswith("void rocprim") else name[:60]
         return name[:90]
 
