@@ -60,6 +60,36 @@ def test_sweep_replay(ctx, name, seed):
     assert np.array_equal(dd.view(np.uint64), dd_o.view(np.uint64))
 
 
+@pytest.mark.parametrize("base", ["multi", "rgg", "dense"])
+def test_signed_zero_weights(ctx, base):
+    """Weights of -0.0 and +0.0: a new per-community counter starts as its first weight and
+    the own community's from 0.0 (louvain.cpp:2456-2462), so counters and clusterWeight keep
+    the sign of zero exactly as the oracle does."""
+    g0 = graphs.graph(base)
+    w = g0.edges["weight"].copy()
+    k = np.arange(len(w))
+    w[k % 3 == 0] = -0.0
+    w[k % 7 == 0] = 0.0
+    g = po.make_graph(g0.offs, g0.edges["tail"], w)
+    dg = ctx.upload(vite_amd.CSR(g.offs, g.edges))
+    for seed in (None, 1, 2):
+        if seed is None:
+            vdeg, c = po.init_degrees(g)
+            curr, csize, cdeg = np.arange(g.nv), np.ones(g.nv, np.int64), vdeg.copy()
+        else:
+            curr, csize, cdeg, vdeg, c = graphs.random_state(g, seed)
+        t_o, cw_o, ds_o, dd_o = po.sweep(g, curr, csize, cdeg, vdeg, c)
+        t, cw, ds, dd = dg.sweep(curr, csize, cdeg, vdeg, c)
+        assert np.array_equal(t, t_o)
+        assert np.array_equal(cw.view(np.uint64), cw_o.view(np.uint64))
+        assert np.array_equal(ds, ds_o)
+        assert np.array_equal(dd.view(np.uint64), dd_o.view(np.uint64))
+    q_o, cv_o, it_o = po.louvain_phase(g)
+    q, cv, it = dg.louvain_phase(sequential=True)
+    assert it == it_o and q == q_o
+    assert np.array_equal(cv, cv_o)
+
+
 @pytest.mark.parametrize("name", graphs.ALL)
 def test_modularity(ctx, name):
     g = graphs.graph(name)

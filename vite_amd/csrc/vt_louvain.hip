@@ -389,7 +389,7 @@ template <int G>
 struct GroupTable {
     using mask_t = typename MaskT<G>::type;
     int32_t* keys;    // CAP slots, -1 = empty
-    double* vals;     // CAP, sentinel = no value yet
+    double* vals;     // CAP, -0.0 = no value yet (the additive identity)
     mask_t* masks;    // CAP: lanes of the current chunk holding the slot's community
     uint8_t* occ;     // 2G: occupied slots in insertion order
     double* wst;      // G staged chunk weights; wst[-1] holds -0.0
@@ -441,18 +441,26 @@ __device__ __forceinline__ void fold_chunk(const GroupTable<G>& t, double* ecc_c
     const mask_t m = *mp;
     wave_sync();
     if (valid && (m & below) == 0) {
+        // An empty slot holds -0.0, and -0.0 + w == w for every w, so a new counter starts
+        // as its first weight.  The vertex's own community starts from 0.0 + w: adding +0.0
+        // turns an empty slot's -0.0 into +0.0 and leaves a running sum unchanged (a sum
+        // that started from +0.0 is never -0.0 under round-to-nearest).
         double acc = t.vals[slot];
-        acc = is_sent(acc) ? (c == cc ? 0.0 + w : w) : acc + w;
+        if (c == cc) acc = acc + 0.0;
+        acc = acc + w;
         acc = fold_rest<mask_t, VT_FOLD_UNROLL>(acc, (mask_t)(m & (m - 1)), t.wst);
         t.vals[slot] = acc;
         t.masks[slot] = 0;
         if (c == cc) *ecc_cell = acc;
     }
-    mask_t sm = (mask_t)((__ballot(self) >> gbase) & gmask);
-    while (sm) {
-        const int k = (G == 64) ? __builtin_ctzll((unsigned long long)sm) : __builtin_ctz((unsigned)sm);
-        sm &= sm - 1;
-        selfLoop = selfLoop + t.wst[k];
+    const unsigned long long sb = __ballot(self);
+    if (sb) {  // wave-uniform: most chunks hold no self-loop
+        mask_t sm = (mask_t)((sb >> gbase) & gmask);
+        while (sm) {
+            const int k = (G == 64) ? __builtin_ctzll((unsigned long long)sm) : __builtin_ctz((unsigned)sm);
+            sm &= sm - 1;
+            selfLoop = selfLoop + t.wst[k];
+        }
     }
     wave_sync();
 }
@@ -634,7 +642,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                 const int32_t ck = *kp;
                 const double ey = *vp;
                 *kp = -1;
-                *vp = sent_value();
+                *vp = -0.0;
                 const bool cand = ok & ((int64_t)ck != cc);
                 const double ay = a.cdeg[cand ? ck : (int32_t)cc];
                 const double gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
@@ -735,7 +743,7 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
         double* wst = (double*)(lds + L::wst);
         for (int i = threadIdx.x; i < NG * CAP; i += BLOCK) {
             keys[i] = -1;
-            vals[i] = sent_value();
+            vals[i] = -0.0;
             masks[i] = 0;
         }
         if (threadIdx.x < NG) wst[threadIdx.x * (G + 1)] = -0.0;
