@@ -1714,7 +1714,8 @@ struct Phase {
     }
 
     // Sorts vertices into (color, degree bucket) lists and, with colors, the per-color event
-    // order; per color, plans one fused launch for buckets 0-4.
+    // order; per color, plans one block range for buckets 0-3 (run as two launches: bucket 2
+    // alone, buckets 0, 1 and 3 together).
     void build_lists(const int64_t* d_colors, int64_t num_color, bool want_corder)
     {
         const int64_t ncol = d_colors && !want_corder ? num_color : 1;
