@@ -278,7 +278,6 @@ typedef struct vt_server vt_server;
 /* Bytes the server reserves for a request (an upper bound on what its job allocates). */
 uint64_t vt_request_reservation(const vt_request* req);
 int vt_server_create(int device, uint64_t hbm_budget_bytes, int32_t slots, vt_server** out);
-/* Queues a request; VT_ERR_BUDGET if its reservation exceeds the whole budget. */
 /* Small-request batching: a slot that admits a request of at most max_n vertices (plain
  * Louvain: no coloring, vertex order or early termination) also takes up to max_batch - 1
  * such requests with the same run parameters queued right behind it, as far as their
@@ -286,9 +285,14 @@ int vt_server_create(int device, uint64_t hbm_budget_bytes, int32_t slots, vt_se
  * request's vt_run alone.  The batched job reserves twice its members' reservations (their
  * graphs and the union).  Default: max_batch 32, max_n 65536; max_batch 1 turns it off. */
 int vt_server_set_batching(vt_server* s, int32_t max_batch, int64_t max_n);
+/* Queues a request; VT_ERR_BUDGET if its reservation exceeds the whole budget.  Submit and
+ * wait may be called from several host threads at once. */
 int vt_server_submit(vt_server* s, const vt_request* req, int64_t* ticket);
-/* Blocks until the request has finished; returns its status.  Each ticket is waited once. */
+/* Blocks until the request has finished; returns its status.  Each ticket is waited once:
+ * a second wait on a ticket already being waited on returns VT_ERR_STATE, and one on a
+ * ticket already collected VT_ERR_ARG. */
 int vt_server_wait(vt_server* s, int64_t ticket, vt_run_result* res, vt_request_info* info);
+/* Message of the calling thread's last failed vt_server_* call. */
 const char* vt_server_last_error(const vt_server* s);
 /* Finishes every queued request, then stops the slots. */
 int vt_server_destroy(vt_server* s);

@@ -117,3 +117,49 @@ def test_oversized_request_rejected(ctx):
             srv.submit("rgg", 1, 1 << 20, 1.0)
         r, info = srv.wait(srv.submit("rgg", 2, 300, 0.0))
         assert info["status"] == 0 and r.final_q > 0
+
+
+def test_threads_submit_and_wait(ctx):
+    """Submit and wait from several host threads at once, with the same results as one
+    thread gets; a ticket is collected once: a wait on a ticket another thread is waiting on
+    is refused with VT_ERR_STATE, and one on a collected ticket with VT_ERR_ARG."""
+    import threading
+    import time
+
+    reqs = _requests(seed=8, count=12)
+    with vite_amd.Server(slots=3) as srv:
+        tickets = [None] * len(reqs)
+        results = [None] * len(reqs)
+
+        def submit(k):
+            tickets[k] = srv.submit(*reqs[k])
+
+        def wait(k):
+            results[k] = srv.wait(tickets[k])
+
+        for fn in (submit, wait):
+            threads = [threading.Thread(target=fn, args=(k,)) for k in range(len(reqs))]
+            for t in threads:
+                t.start()
+            for t in threads:
+                t.join()
+        for r, info in results:
+            assert info["status"] == 0
+        with pytest.raises(vite_amd.VtError) as e:
+            srv.wait(tickets[0])
+        assert e.value.code == vite_amd.VT_ERR_ARG
+
+        big = srv.submit("rgg", 77, 1 << 18, 1.0)
+        first = []
+        waiter = threading.Thread(target=lambda: first.append(srv.wait(big)))
+        waiter.start()
+        time.sleep(0.05)
+        with pytest.raises(vite_amd.VtError) as e:
+            srv.wait(big)
+        # refused while the other thread waits, or already collected by it
+        assert e.value.code in (vite_amd.VT_ERR_STATE, vite_amd.VT_ERR_ARG)
+        waiter.join()
+        assert first and first[0][1]["status"] == 0
+    with vite_amd.Server(slots=1) as one:
+        for r, (rm, _) in zip(reqs, results):
+            _same(rm, one.wait(one.submit(*r))[0])

@@ -30,6 +30,7 @@ namespace {
 
 struct Job {
     int64_t ticket = 0;
+    bool claimed = false;  // a vt_server_wait is blocked on it
     vt_graph* g = nullptr;
     vt_request req{};
     uint64_t reserve = 0;
@@ -55,7 +56,6 @@ struct vt_server {
     int32_t max_batch = 32;          // requests per batched job (1: no batching)
     int64_t batch_max_n = 1 << 16;   // largest graph a batched job takes
     bool stop = false;
-    std::string err;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -228,6 +228,12 @@ void worker(vt_server* s, int slot)
 
 using namespace vt;
 
+namespace {
+// Errors of the vt_server_* calls, per calling thread: submit and wait may run on several
+// host threads at once.
+thread_local std::string tl_server_err;
+}  // namespace
+
 extern "C" {
 
 uint64_t vt_request_reservation(const vt_request* r)
@@ -276,19 +282,22 @@ int vt_server_submit(vt_server* s, const vt_request* req, int64_t* ticket)
     if (!s || !req || !ticket) return VT_ERR_ARG;
     if (req->n < 1 || (req->kind != VT_REQ_RGG && req->kind != VT_REQ_BA) ||
         (req->kind == VT_REQ_BA && req->param < 1.0)) {
-        s->err = "vt_server_submit: bad request";
+        tl_server_err = "vt_server_submit: bad request";
         return VT_ERR_ARG;
     }
     auto j = std::make_unique<Job>();
     j->req = *req;
     j->reserve = vt_request_reservation(req);
     if (j->reserve > s->budget) {
-        s->err = "vt_server_submit: request needs " + std::to_string(j->reserve) + " bytes, more than the budget";
+        tl_server_err = "vt_server_submit: request needs " + std::to_string(j->reserve) + " bytes, more than the budget";
         return VT_ERR_BUDGET;
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop) return VT_ERR_STATE;
+        if (s->stop) {
+            tl_server_err = "vt_server_submit: the server is stopping";
+            return VT_ERR_STATE;
+        }
         j->ticket = s->next_ticket++;
         j->info.submit_s = s->now();
         j->info.reserved_bytes = j->reserve;
@@ -308,23 +317,30 @@ int vt_server_wait(vt_server* s, int64_t ticket, vt_run_result* res, vt_request_
         std::unique_lock<std::mutex> lk(s->mu);
         auto it = s->jobs.find(ticket);
         if (it == s->jobs.end()) {
-            s->err = "vt_server_wait: unknown ticket";
+            tl_server_err = "vt_server_wait: unknown or already collected ticket";
             return VT_ERR_ARG;
         }
         Job* p = it->second.get();
+        if (p->claimed) {
+            tl_server_err = "vt_server_wait: another thread is already waiting on this ticket";
+            return VT_ERR_STATE;
+        }
+        p->claimed = true;
         s->cv_done.wait(lk, [&] { return p->done; });
+        // std::map iterators stay valid while other tickets come and go, and only the
+        // thread that claimed this ticket erases it
         j = std::move(it->second);
         s->jobs.erase(it);
     }
     if (res) *res = j->res;
     if (info) *info = j->info;
-    if (j->info.status != VT_OK) s->err = j->err;
+    if (j->info.status != VT_OK) tl_server_err = j->err;
     return j->info.status;
 }
 
 const char* vt_server_last_error(const vt_server* s)
 {
-    return s ? s->err.c_str() : "null server";
+    return s ? tl_server_err.c_str() : "null server";
 }
 
 int vt_server_destroy(vt_server* s)
