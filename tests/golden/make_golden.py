@@ -66,6 +66,7 @@ GRAPH_SPECS = {
     "hub": {"kind": "named", "name": "hub"},
     "multi": {"kind": "named", "name": "multi"},
     "grid": {"kind": "named", "name": "grid"},
+    "karate": {"kind": "named", "name": "karate"},
 }
 
 

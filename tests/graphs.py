@@ -82,6 +82,21 @@ def _dense_geometric(seed=21, n=2400, mean_deg=70.0, extra=3000):
     return po.from_edge_list(n, pairs, wts[first])
 
 
+# Zachary's karate club (34 members, 78 friendships; W. W. Zachary, J. Anthropol. Res. 33,
+# 1977): for each member (1-based), the higher-numbered members it is tied to.
+_KARATE = {1: [2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 14, 18, 20, 22, 32], 2: [3, 4, 8, 14, 18, 20, 22, 31],
+           3: [4, 8, 9, 10, 14, 28, 29, 33], 4: [8, 13, 14], 5: [7, 11], 6: [7, 11, 17], 7: [17],
+           9: [31, 33, 34], 10: [34], 14: [34], 15: [33, 34], 16: [33, 34], 19: [33, 34], 20: [34],
+           21: [33, 34], 23: [33, 34], 24: [26, 28, 30, 33, 34], 25: [26, 28, 32], 26: [32], 27: [30, 34],
+           28: [34], 29: [32, 34], 30: [33, 34], 31: [33, 34], 32: [33, 34], 33: [34]}
+
+
+def _karate():
+    """Zachary's karate club with unit weights, 0-based ids."""
+    pairs = [(a - 1, b - 1) for a, bs in _KARATE.items() for b in bs]
+    return po.from_edge_list(34, pairs)
+
+
 @functools.lru_cache(maxsize=None)
 def graph(name: str) -> po.Graph:
     if name == "rgg":
@@ -100,10 +115,12 @@ def graph(name: str) -> po.Graph:
         return _unit_grid()
     if name == "dense":
         return _dense_geometric()
+    if name == "karate":
+        return _karate()
     raise KeyError(name)
 
 
-ALL = ["rgg", "rgg_plain", "ba", "vite_rgg", "hub", "multi", "grid", "dense"]
+ALL = ["rgg", "rgg_plain", "ba", "vite_rgg", "hub", "multi", "grid", "dense", "karate"]
 
 
 def random_state(g: po.Graph, seed: int = 0):

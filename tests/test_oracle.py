@@ -337,3 +337,15 @@ def test_oracle_vs_reference_sweep(name, seed):
     b = po.ref_sweep(g, *st)
     for x, y in zip(a, b):
         assert np.array_equal(np.asarray(x).view(np.uint64), np.asarray(y).view(np.uint64))
+
+
+def test_karate_club_known_answer():
+    """Zachary's karate club (public data): the whole run ends inside the band Louvain
+    reaches on this graph, below the proven optimum 0.4197896 (Brandes et al., IEEE TKDE 20,
+    2008), whatever the flags; the compiled reference gives the same Q (golden runs)."""
+    g = graphs.graph("karate")
+    assert (g.nv, g.ne) == (34, 156)
+    for kw in [dict(), dict(threshold_cycling=True), dict(coloring=True, et_type=2, et_delta=0.25)]:
+        r = po.run(g, **kw)
+        assert 0.40 < r.final_q < 0.4197896
+        assert r.level_nv[0] == 34 and r.level_nv[-1] < 34
