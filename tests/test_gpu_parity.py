@@ -201,7 +201,7 @@ RUNS = [dict(), dict(threshold_cycling=True), dict(coloring=True, et_type=2, et_
                                                                    et_delta=0.5), dict(one_phase=True)]
 
 
-@pytest.mark.parametrize("name", ["rgg", "ba", "vite_rgg", "hub", "grid"])
+@pytest.mark.parametrize("name", ["rgg", "ba", "vite_rgg", "hub", "grid", "karate"])
 @pytest.mark.parametrize("kw", RUNS, ids=[str(i) for i in range(len(RUNS))])
 def test_run_exact(ctx, name, kw):
     """The whole phase loop (main.cpp:242-513): every Louvain call's return value and
