@@ -14,7 +14,10 @@ reference's own code at one MPI rank and one OpenMP thread:
   * sweeps:  distExecuteLouvainIteration over all vertices from seeded states
              (targets / clusterWeight / localCupdate hashes);
   * colors:  distColoringMultiHashMinMax colorings;
-  * vite_rgg: generateInMemGraph graphs (Vite's LCG RGG) as CSR hashes.
+  * vite_rgg: generateInMemGraph graphs (Vite's LCG RGG) as CSR hashes;
+  * communities: the per-vertex community file main() writes with -o (main.cpp:380-425,
+             543-572), as a hash of the int64 array, with its community count and the
+             run's final Q.
 Graph inputs are recorded as generator specs plus a hash of the generated CSR, so the
 fixture stays a few kilobytes; the hash pins the generator output the vectors were made on.
 """
@@ -82,13 +85,16 @@ def build(spec):
 
 RUN_FLAGS = [[], ["-i"], ["-c", "8", "-t", "2", "-a", "0.25"], ["-t", "1"], ["-d", "8", "-t", "3"],
              ["-c", "8 true", "-t", "4", "-a", "0.5"], ["-p"], ["-t", "2", "-a", "0.9"], ["-i", "-c", "4"]]
+# flag sets whose community file (-o: <input>.communities, main.cpp:380-425,543-572) is recorded
+OUTPUT_FLAGS = [[], ["-i"], ["-c", "8", "-t", "2", "-a", "0.25"], ["-p"]]
 
 
 def main():
     assert po.ref_available(), "build oracle/_ref first (make ref)"
     assert os.environ.get("OMP_NUM_THREADS") == "1", "run with OMP_NUM_THREADS=1"
     out = {"about": "Vectors from the compiled reference at 1 MPI rank, 1 OpenMP thread; see make_golden.py",
-           "graphs": {}, "runs": [], "sweeps": [], "colors": [], "vite_rgg": [], "rebuilds": []}
+           "graphs": {}, "runs": [], "sweeps": [], "colors": [], "vite_rgg": [], "rebuilds": [],
+           "communities": []}
     tmp = tempfile.mkdtemp()
     for name, spec in GRAPH_SPECS.items():
         g = build(spec)
@@ -102,6 +108,13 @@ def main():
             rebuilds = [{"nv": int(r["nv"]), "ne": int(r["ne"]), "csr_fnv": r["csr_fnv"]}
                         for r in recs if r["kind"] == "rebuild"]
             out["runs"].append({"graph": name, "flags": flags, "calls": calls, "rebuilds": rebuilds})
+        for flags in OUTPUT_FLAGS:
+            recs, _ = po.ref_trace_run(path, ["-o"] + flags)
+            comm = np.loadtxt(path + ".communities", dtype=np.int64, ndmin=1)
+            os.remove(path + ".communities")
+            out["communities"].append({"graph": name, "flags": flags, "nv": int(len(comm)),
+                                       "count": int(len(np.unique(comm))), "sha256": arr_sha(comm),
+                                       "final_q_hex": float([r for r in recs if r["kind"] != "rebuild"][-1]["q"]).hex()})
         # sweeps from seeded states
         for seed in [None, 7]:
             if seed is None:

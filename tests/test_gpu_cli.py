@@ -1,0 +1,59 @@
+"""GPU: the command-line driver end to end against the reference's own main().
+
+A Vite .bin file goes in (the reference's binary graph format, distgraph.cpp:119-253); the
+per-vertex community file that -o writes (<input>.communities, renumbered per level as in
+main.cpp:380-425 and written as in main.cpp:543-572) and the final modularity come out.
+Both must equal what the compiled reference wrote for the same file and flags
+(tests/golden/reference_runs.json, "communities").  Sums follow the reference order
+(VITE_AMD_SEQUENTIAL_SUMS=1), so every decision and so the partition is identical.
+"""
+import hashlib
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import graphs
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "vite_amd", "bin", "vite_amd_cluster")
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_runs.json")))
+CASES = GOLDEN["communities"]
+
+
+def _build(spec):
+    if spec["kind"] == "rgg":
+        return po.gen_rgg(spec["seed"], spec["n"], spec["extra_pct"])
+    if spec["kind"] == "ba":
+        return po.gen_ba(spec["seed"], spec["n"], spec["m"])
+    if spec["kind"] == "vite_rgg":
+        return po.gen_vite_rgg(spec["n"])
+    return graphs.graph(spec["name"])
+
+
+@pytest.mark.parametrize("idx", range(len(CASES)),
+                         ids=[f"{c['graph']}:{' '.join(c['flags']) or 'default'}" for c in CASES])
+def test_cli_community_file_matches_reference(tmp_path, idx):
+    case = CASES[idx]
+    g = _build(GOLDEN["graphs"][case["graph"]]["spec"])
+    path = str(tmp_path / (case["graph"] + ".bin"))
+    po.write_bin(path, g)
+    env = dict(os.environ, VITE_AMD_SEQUENTIAL_SUMS="1")
+    out = subprocess.run([CLI, "-f", path, "-o"] + case["flags"], capture_output=True, text=True, env=env,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    comm = np.loadtxt(path + ".communities", dtype=np.int64, ndmin=1)
+    assert len(comm) == case["nv"]
+    assert len(np.unique(comm)) == case["count"]
+    assert hashlib.sha256(np.ascontiguousarray(comm).tobytes()).hexdigest() == case["sha256"]
+    # the last Louvain call's Q; after a one-phase run (-p) the loop has already reset
+    # currMod to -1, as the reference's MODS line shows (main.cpp:474-482, 526-528)
+    expect = -1.0 if "-p" in case["flags"] else float.fromhex(case["final_q_hex"])
+    m = re.search(r"^Final modularity: (\S+)$", out.stdout, re.M)
+    assert m and float(m.group(1)) == expect
