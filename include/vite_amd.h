@@ -18,6 +18,7 @@
  *   vt_rebuild             distbuildNextLevelGraph (rebuild.hpp:109-112; rebuild.cpp:480-504)
  *   vt_color               distColoringMultiHashMinMax (coloring.hpp:86; coloring.cpp:52-118)
  *   vt_gen_vite_rgg        generateInMemGraph at one rank (distgraph.hpp:116; distgraph.cpp:391-983)
+ *   vt_compare_communities compare_communities / compute_gini_coeff (compare.hpp; compare.cpp:58-337)
  *   vt_gen_rgg, vt_gen_ba  the workload generators (no reference counterpart: Vite's RGG uses
  *                          another radius and an O(n^2) pair test; its BA needs NetworKit)
  *   vt_run                 the phase loop of main.cpp:242-513
@@ -173,6 +174,24 @@ int vt_rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* cvect, vt_graph** 
 /* ---- coloring ----------------------------------------------------------------------- */
 int vt_color(vt_ctx* ctx, const vt_graph* g, int64_t nhash, int32_t target_pct,
              int32_t single_iter, int64_t* colors_out, int64_t* next_color_out);
+
+/* ---- ground-truth comparison ------------------------------------------------------ */
+/* compare_communities (compare.cpp:58-307) with compute_gini_coeff (compare.cpp:310-337):
+ * pair counts of a computed partition against a ground truth, both host arrays of n
+ * community ids >= 0 (ids need not be dense: a partition has max id + 1 communities, empty
+ * ones included, as in the reference).  The counts are exact; precision, recall, F-score
+ * and the Gini coefficients use the reference's expressions and its one-thread order. */
+typedef struct {
+    int64_t n;
+    int64_t nc_truth, nc_found;  /* max id + 1 of each partition                           */
+    int64_t same_same;           /* pairs together in both (true positives)                */
+    int64_t same_diff;           /* together in the truth, apart in found (false negatives) */
+    int64_t diff_same;           /* apart in the truth, together in found (false positives) */
+    double precision, recall, f_score;
+    double gini_truth, gini_found;
+} vt_compare_stats;
+int vt_compare_communities(vt_ctx* ctx, int64_t n, const int64_t* truth, const int64_t* found,
+                           vt_compare_stats* out);
 
 /* ---- generators (device) --------------------------------------------------------- */
 /* Seeded random geometric graph in [0,1]^2 with radius sqrt(2 ln n/(pi n)) and extra_pct

@@ -287,6 +287,25 @@ def run(g: Graph, threshold=1e-6, threshold_cycling=False, et_type=0, et_delta=1
                      res.final_q, res.total_iters, res.num_colors, [int(x) for x in res.call_cvect_fnv[:m]])
 
 
+class _CompareStats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nc_truth", C.c_int64), ("nc_found", C.c_int64), ("same_same", C.c_int64),
+                ("same_diff", C.c_int64), ("diff_same", C.c_int64), ("precision", C.c_double),
+                ("recall", C.c_double), ("f_score", C.c_double), ("gini_truth", C.c_double),
+                ("gini_found", C.c_double)]
+
+
+def compare(truth, found) -> dict:
+    """compare_communities + compute_gini_coeff (compare.cpp:58-337) at one thread."""
+    t = np.ascontiguousarray(truth, np.int64)
+    f = np.ascontiguousarray(found, np.int64)
+    assert len(t) == len(f)
+    st = _CompareStats()
+    rc = _lib().or_compare(C.c_int64(len(t)), _p(t), _p(f), C.byref(st))
+    if rc != 0:
+        raise ValueError(f"or_compare: {rc}")
+    return {k: getattr(st, k) for k, _ in _CompareStats._fields_}
+
+
 def fnv_i64(v) -> int:
     v = np.ascontiguousarray(v, np.int64)
     return int(_lib().or_fnv_i64(_p(v), C.c_int64(len(v))))

@@ -1001,3 +1001,90 @@ int or_gen_vite_rgg(int64_t n, or_graph* out)
     free(fill); free(deg); free(rnums); free(drand); free(Y);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------ */
+/* Ground-truth comparison                                                          */
+/* ------------------------------------------------------------------------------ */
+
+/* Members of each community id in [0, nc) grouped in vertex order (the counting layout of
+ * compare.cpp:97-140 at one thread); sizes[c] is community c's member count. */
+static void group_members(int64_t n, const int64_t* c, int64_t nc, int64_t* ptr, int64_t* idx, int64_t* sizes)
+{
+    int64_t* added = (int64_t*)calloc((size_t)nc, sizeof(int64_t));
+    for (int64_t i = 0; i <= nc; i++) ptr[i] = 0;
+    for (int64_t i = 0; i < n; i++) ptr[c[i] + 1]++;
+    for (int64_t i = 0; i < nc; i++) sizes[i] = ptr[i + 1];
+    for (int64_t i = 0; i < nc; i++) ptr[i + 1] += ptr[i];
+    for (int64_t i = 0; i < n; i++) idx[ptr[c[i]] + added[c[i]]++] = i;
+    free(added);
+}
+
+static int cmp_i64(const void* a, const void* b)
+{
+    const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* compute_gini_coeff (compare.cpp:310-337): sizes sorted ascending, then the one-thread left
+ * folds of (i+1)*size[i] and size[i]; numColors is an int there. */
+static double gini_coeff(int64_t* sizes, int numColors)
+{
+    qsort(sizes, (size_t)numColors, sizeof(int64_t), cmp_i64);
+    double numFunc = 0.0, denFunc = 0.0;
+    for (int64_t i = 0; i < numColors; i++) {
+        numFunc = numFunc + (double)((i + 1) * sizes[i]);
+        denFunc = denFunc + (double)sizes[i];
+    }
+    return ((2 * numFunc) / (numColors * denFunc)) - ((double)(numColors + 1) / (double)numColors);
+}
+
+/* compare_communities (compare.cpp:58-307): C1 = truth, C2 = found.  For every pair of
+ * members of a C1 community, same-same if C2 also joins them, else same-diff; for every
+ * pair of members of a C2 community, diff-same if C1 separates them.  Then precision,
+ * recall and F-score as at compare.cpp:266-272. */
+int or_compare(int64_t n, const int64_t* truth, const int64_t* found, or_compare_stats* out)
+{
+    if (n < 1) return -1;
+    int64_t nc1 = -1, nc2 = -1;
+    for (int64_t i = 0; i < n; i++) {
+        if (truth[i] < 0 || found[i] < 0) return -1;
+        if (truth[i] > nc1) nc1 = truth[i];
+        if (found[i] > nc2) nc2 = found[i];
+    }
+    nc1++;
+    nc2++;
+    int64_t* ptr1 = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nc1 + 1));
+    int64_t* idx1 = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t* size1 = (int64_t*)malloc(sizeof(int64_t) * (size_t)nc1);
+    int64_t* ptr2 = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nc2 + 1));
+    int64_t* idx2 = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t* size2 = (int64_t*)malloc(sizeof(int64_t) * (size_t)nc2);
+    group_members(n, truth, nc1, ptr1, idx1, size1);
+    group_members(n, found, nc2, ptr2, idx2, size2);
+    int64_t ss = 0, sd = 0, ds = 0, agree = 0;
+    for (int64_t c = 0; c < nc1; c++)
+        for (int64_t i = ptr1[c]; i < ptr1[c + 1]; i++)
+            for (int64_t j = i + 1; j < ptr1[c + 1]; j++) {
+                if (found[idx1[i]] == found[idx1[j]]) ss++;
+                else sd++;
+            }
+    for (int64_t c = 0; c < nc2; c++)
+        for (int64_t i = ptr2[c]; i < ptr2[c + 1]; i++)
+            for (int64_t j = i + 1; j < ptr2[c + 1]; j++) {
+                if (truth[idx2[i]] == truth[idx2[j]]) agree++;
+                else ds++;
+            }
+    out->n = n;
+    out->nc_truth = nc1;
+    out->nc_found = nc2;
+    out->same_same = ss;
+    out->same_diff = sd;
+    out->diff_same = ds;
+    out->precision = (double)ss / (double)(ss + ds);
+    out->recall = (double)ss / (double)(ss + sd);
+    out->f_score = 2.0 * ((out->precision * out->recall) / (out->precision + out->recall));
+    out->gini_truth = gini_coeff(size1, (int)nc1);
+    out->gini_found = gini_coeff(size2, (int)nc2);
+    free(ptr1); free(idx1); free(size1); free(ptr2); free(idx2); free(size2);
+    return agree == ss ? 0 : -2;  /* the reference asserts the two counts agree */
+}

@@ -126,6 +126,16 @@ int or_gen_ba(uint64_t seed, int64_t n, int64_t m, or_graph* out);
 int or_gen_vite_rgg(int64_t n, or_graph* out);
 uint32_t or_seed_seq_1(uint32_t s);
 
+/* ---- Ground-truth comparison ---- */
+typedef struct {
+    int64_t n, nc_truth, nc_found;
+    int64_t same_same, same_diff, diff_same;
+    double precision, recall, f_score, gini_truth, gini_found;
+} or_compare_stats;
+/* compare_communities (compare.cpp:58-307) and compute_gini_coeff (compare.cpp:310-337)
+ * at one thread: per-community member lists, then every member pair. -1 on bad input. */
+int or_compare(int64_t n, const int64_t* truth, const int64_t* found, or_compare_stats* out);
+
 /* FNV-1a of an int64 vector (for compact fixtures). */
 uint64_t or_fnv_i64(const int64_t* v, int64_t n);
 

@@ -17,7 +17,10 @@ reference's own code at one MPI rank and one OpenMP thread:
   * vite_rgg: generateInMemGraph graphs (Vite's LCG RGG) as CSR hashes;
   * communities: the per-vertex community file main() writes with -o (main.cpp:380-425,
              543-572), as a hash of the int64 array, with its community count and the
-             run's final Q.
+             run's final Q;
+  * ground_truth: main() with -g FILE [-z] (compare.cpp:58-337): the run's community file
+             and the comparison statistics it printed (pair counts, precision, recall,
+             F-score, Gini coefficients; floats as hex).
 Graph inputs are recorded as generator specs plus a hash of the generated CSR, so the
 fixture stays a few kilobytes; the hash pins the generator output the vectors were made on.
 """
@@ -88,13 +91,44 @@ RUN_FLAGS = [[], ["-i"], ["-c", "8", "-t", "2", "-a", "0.25"], ["-t", "1"], ["-d
 # flag sets whose community file (-o: <input>.communities, main.cpp:380-425,543-572) is recorded
 OUTPUT_FLAGS = [[], ["-i"], ["-c", "8", "-t", "2", "-a", "0.25"], ["-p"]]
 
+# ground-truth comparisons (-g FILE [-z], main.cpp:119-137,574-580; compare.cpp:58-337):
+# (graph, truth kind, one-based file, run flags)
+TRUTH_CASES = [("karate", "factions", False, []), ("karate", "factions", True, ["-i"]),
+               ("rgg_small", "blocks", False, []), ("rgg_small", "sparse_blocks", True, ["-p"]),
+               ("ba_small", "blocks", False, ["-i"]), ("grid", "sparse_blocks", False, [])]
+
+STAT_LINES = {"same_same": "Same-Same (True positive)  :", "same_diff": "Same-Diff (False negative) :",
+              "diff_same": "Diff-Same (False positive) :", "nc_truth": "#communities in C1 :",
+              "nc_found": "#communities in C2 :", "precision": "Precision :", "recall": "Recall    :",
+              "f_score": "F-score   :", "gini_truth": "Gini coefficient, C1  :",
+              "gini_found": "Gini coefficient, C2  :"}
+
+
+def truth_vector(kind, nv):
+    """A ground truth: the karate club's factions, blocks of 37 consecutive vertices, or
+    blocks of 50 with ids 3b + 1 (ids with no members in between)."""
+    if kind == "factions":
+        return graphs.karate_factions()
+    v = np.arange(nv, dtype=np.int64)
+    return v // 37 if kind == "blocks" else 3 * (v // 50) + 1
+
+
+def parse_stats(text):
+    out = {}
+    for key, label in STAT_LINES.items():
+        line = next(x for x in text.splitlines() if x.startswith(label))
+        val = line[len(label):].split()[0]
+        out[key] = int(val) if key in ("same_same", "same_diff", "diff_same", "nc_truth", "nc_found") else \
+            float(val).hex()
+    return out
+
 
 def main():
     assert po.ref_available(), "build oracle/_ref first (make ref)"
     assert os.environ.get("OMP_NUM_THREADS") == "1", "run with OMP_NUM_THREADS=1"
     out = {"about": "Vectors from the compiled reference at 1 MPI rank, 1 OpenMP thread; see make_golden.py",
            "graphs": {}, "runs": [], "sweeps": [], "colors": [], "vite_rgg": [], "rebuilds": [],
-           "communities": []}
+           "communities": [], "ground_truth": []}
     tmp = tempfile.mkdtemp()
     for name, spec in GRAPH_SPECS.items():
         g = build(spec)
@@ -134,6 +168,19 @@ def main():
         r = po.ref_rebuild(g, cv)
         out["rebuilds"].append({"graph": name, "cvect_sha256": arr_sha(cv.astype(np.int64)), "nv": r.nv,
                                 "ne": r.ne, "csr_sha256": csr_sha(r)})
+    for name, kind, one_based, flags in TRUTH_CASES:
+        g = build(GRAPH_SPECS[name])
+        path = os.path.join(tmp, name + ".bin")
+        truth = truth_vector(kind, g.nv)
+        tpath = os.path.join(tmp, f"{name}.{kind}.truth")
+        with open(tpath, "w") as f:
+            for v, c in enumerate(truth.tolist()):
+                f.write(f"{v + 1 if one_based else v} {c + 1 if one_based else c}\n")
+        _, text = po.ref_trace_run(path, ["-o", "-g", tpath] + (["-z"] if one_based else []) + flags)
+        comm = np.loadtxt(path + ".communities", dtype=np.int64, ndmin=1)
+        os.remove(path + ".communities")
+        out["ground_truth"].append({"graph": name, "truth": kind, "one_based": one_based, "flags": flags,
+                                    "communities": comm.tolist(), "stats": parse_stats(text)})
     for n in [256, 1000, 4096]:
         g = po.ref_gen_rgg(n)
         out["vite_rgg"].append({"n": n, "nv": g.nv, "ne": g.ne, "csr_sha256": csr_sha(g)})

@@ -91,6 +91,16 @@ _KARATE = {1: [2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 14, 18, 20, 22, 32], 2: [3, 4
            28: [34], 29: [32, 34], 30: [33, 34], 31: [33, 34], 32: [33, 34], 33: [34]}
 
 
+# The club's split after the conflict (Zachary's faction membership, 1-based): members who
+# went with the instructor; the rest went with the officer.
+KARATE_INSTRUCTOR = [1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 13, 14, 17, 18, 20, 22]
+
+
+def karate_factions():
+    """Ground truth for the karate club: 0 = instructor's faction, 1 = officer's."""
+    return np.array([0 if v + 1 in KARATE_INSTRUCTOR else 1 for v in range(34)], dtype=np.int64)
+
+
 def _karate():
     """Zachary's karate club with unit weights, 0-based ids."""
     pairs = [(a - 1, b - 1) for a, bs in _KARATE.items() for b in bs]

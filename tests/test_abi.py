@@ -79,6 +79,7 @@ def test_host_layer_exports_reference_entry_points():
     (["-f", "x.bin", "-p", "-i"], "Cannot turn on threshold-cycling"),
     ([], "Must specify a binary file name with -f"),
     (["-f", "x.bin", "-e", "1.0"], "Must specify -n"),
+    (["-f", "x.bin", "-g", ""], "Must specify a ground-truth file name with -g"),
 ])
 def test_cli_rejects_what_the_reference_rejects(args, msg):
     r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=60)
@@ -90,6 +91,14 @@ def test_cli_missing_file_is_fatal(tmp_path):
     r = subprocess.run([CLI, "-f", str(tmp_path / "missing.bin")], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "Error opening file" in r.stderr
+
+
+def test_cli_missing_ground_truth_is_fatal(tmp_path):
+    """The ground truth is read before the graph (main.cpp:119-137), as the reference does."""
+    r = subprocess.run([CLI, "-f", str(tmp_path / "g.bin"), "-g", str(tmp_path / "missing.truth")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "Error opening ground truth file" in r.stderr
 
 
 def test_server_without_gpu_fails_cleanly():

@@ -57,3 +57,48 @@ def test_cli_community_file_matches_reference(tmp_path, idx):
     expect = -1.0 if "-p" in case["flags"] else float.fromhex(case["final_q_hex"])
     m = re.search(r"^Final modularity: (\S+)$", out.stdout, re.M)
     assert m and float(m.group(1)) == expect
+
+
+STAT_LINES = {"same_same": "Same-Same (True positive)  :", "same_diff": "Same-Diff (False negative) :",
+              "diff_same": "Diff-Same (False positive) :", "nc_truth": "#communities in C1 :",
+              "nc_found": "#communities in C2 :", "precision": "Precision :", "recall": "Recall    :",
+              "f_score": "F-score   :", "gini_truth": "Gini coefficient, C1  :",
+              "gini_found": "Gini coefficient, C2  :"}
+TRUTH = GOLDEN["ground_truth"]
+
+
+def _truth_vector(kind, nv):
+    if kind == "factions":
+        return graphs.karate_factions()
+    v = np.arange(nv, dtype=np.int64)
+    return v // 37 if kind == "blocks" else 3 * (v // 50) + 1
+
+
+@pytest.mark.parametrize("idx", range(len(TRUTH)),
+                         ids=[f"{c['graph']}:{c['truth']}:{' '.join(c['flags']) or 'default'}" for c in TRUTH])
+def test_cli_ground_truth_report_matches_reference(tmp_path, idx):
+    """-g FILE [-z]: the comparison report (compare.cpp:274-293) equals the one the compiled
+    reference printed for the same graph, truth file and flags; counts exact, scores to
+    the last bit."""
+    case = TRUTH[idx]
+    g = _build(GOLDEN["graphs"][case["graph"]]["spec"])
+    path = str(tmp_path / (case["graph"] + ".bin"))
+    po.write_bin(path, g)
+    truth = _truth_vector(case["truth"], g.nv)
+    tpath = str(tmp_path / "truth.txt")
+    one = case["one_based"]
+    with open(tpath, "w") as f:
+        for v, c in enumerate(truth.tolist()):
+            f.write(f"{v + 1 if one else v} {c + 1 if one else c}\n")
+    env = dict(os.environ, VITE_AMD_SEQUENTIAL_SUMS="1")
+    out = subprocess.run([CLI, "-f", path, "-g", tpath] + (["-z"] if one else []) + case["flags"],
+                         capture_output=True, text=True, env=env, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    lines = out.stdout.splitlines()
+    for key, label in STAT_LINES.items():
+        val = next(x for x in lines if x.startswith(label))[len(label):].split()[0]
+        want = case["stats"][key]
+        if isinstance(want, int):
+            assert int(val) == want, key
+        else:
+            assert float(val).hex() == want, key

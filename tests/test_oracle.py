@@ -349,3 +349,55 @@ def test_karate_club_known_answer():
         r = po.run(g, **kw)
         assert 0.40 < r.final_q < 0.4197896
         assert r.level_nv[0] == 34 and r.level_nv[-1] < 34
+
+
+def _truth_vector(kind, nv):
+    if kind == "factions":
+        return graphs.karate_factions()
+    v = np.arange(nv, dtype=np.int64)
+    return v // 37 if kind == "blocks" else 3 * (v // 50) + 1
+
+
+def _stats_as_golden(st):
+    return {k: (st[k] if k in ("same_same", "same_diff", "diff_same", "nc_truth", "nc_found") else float(st[k]).hex())
+            for k in ("same_same", "same_diff", "diff_same", "nc_truth", "nc_found", "precision", "recall",
+                      "f_score", "gini_truth", "gini_found")}
+
+
+@pytest.mark.parametrize("idx", range(len(GOLDEN["ground_truth"])))
+def test_golden_ground_truth_compare(golden_graphs, idx):
+    """compare_communities as the compiled reference printed it for -g runs: the oracle,
+    given the same truth and the reference's own community file, gives the same counts and
+    bit-identical precision, recall, F-score and Gini coefficients."""
+    case = GOLDEN["ground_truth"][idx]
+    g = golden_graphs[case["graph"]]
+    truth = _truth_vector(case["truth"], g.nv)
+    st = po.compare(truth, np.array(case["communities"], dtype=np.int64))
+    assert _stats_as_golden(st) == case["stats"]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_compare_pair_counts_brute_force(seed):
+    """Pair counts over every vertex pair, counted directly, and the closed forms of the
+    identical and the all-apart partitions."""
+    import itertools
+    rng = np.random.default_rng(seed)
+    n = 60
+    t = rng.integers(0, 6, n)
+    f = rng.integers(0, 9, n) * 2  # ids with gaps
+    st = po.compare(t, f)
+    ss = sd = ds = 0
+    for i, j in itertools.combinations(range(n), 2):
+        a, b = t[i] == t[j], f[i] == f[j]
+        ss += a and b
+        sd += a and not b
+        ds += b and not a
+    assert (st["same_same"], st["same_diff"], st["diff_same"]) == (ss, sd, ds)
+    assert (st["nc_truth"], st["nc_found"]) == (t.max() + 1, f.max() + 1)
+    same = po.compare(t, t)
+    assert same["diff_same"] == same["same_diff"] == 0 and same["precision"] == same["recall"] == 1.0
+    apart = po.compare(t, np.arange(n))
+    assert apart["same_same"] == 0 and apart["diff_same"] == 0
+    # Gini of equal sizes is 0; of one member in one of k ids it is (k - 1) / k
+    assert po.compare(np.repeat(np.arange(4), 5), np.repeat(np.arange(4), 5))["gini_truth"] == 0.0
+    assert abs(po.compare(np.array([3]), np.array([0]))["gini_truth"] - 3 / 4) < 1e-15

@@ -38,6 +38,13 @@ _i64p = C.POINTER(C.c_int64)
 _f64p = C.POINTER(C.c_double)
 
 
+class _CompareStats(C.Structure):
+    _fields_ = [("n", C.c_int64), ("nc_truth", C.c_int64), ("nc_found", C.c_int64), ("same_same", C.c_int64),
+                ("same_diff", C.c_int64), ("diff_same", C.c_int64), ("precision", C.c_double),
+                ("recall", C.c_double), ("f_score", C.c_double), ("gini_truth", C.c_double),
+                ("gini_found", C.c_double)]
+
+
 class VtError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"vite_amd error {code}: {msg}")
@@ -118,6 +125,7 @@ _SIGS = {
     "vt_init_degrees": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _f64p, _f64p]),
     "vt_rebuild": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, C.POINTER(C.c_void_p)]),
     "vt_color": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, _i64p, _i64p]),
+    "vt_compare_communities": (C.c_int, [C.c_void_p, C.c_int64, _i64p, _i64p, C.c_void_p]),
     "vt_gen_rgg": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_double, C.POINTER(C.c_void_p)]),
     "vt_gen_ba": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
     "vt_gen_vite_rgg": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -273,6 +281,17 @@ class Context:
         h = C.c_void_p()
         self.check(_lib.vt_gen_vite_rgg(self.h, n, x0, C.byref(h)))
         return DeviceGraph(self, h)
+
+    def compare_communities(self, truth, found) -> dict:
+        """Pair counts, precision, recall, F-score and Gini coefficients of a partition
+        against a ground truth (vt_compare_communities)."""
+        t = np.ascontiguousarray(truth, np.int64)
+        f = np.ascontiguousarray(found, np.int64)
+        if len(t) != len(f):
+            raise ValueError("truth and found differ in length")
+        st = _CompareStats()
+        self.check(_lib.vt_compare_communities(self.h, len(t), _p(t), _p(f), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _CompareStats._fields_}
 
     def modularity(self, cw, cdeg, constant: float, sequential: bool = False) -> float:
         cw = np.ascontiguousarray(cw, np.float64)

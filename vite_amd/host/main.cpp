@@ -28,6 +28,8 @@ struct Options {
     bool coloring = false, vertex_order = false, one_phase = false, single_color_iter = false;
     bool generate = false, just_process = false, output_communities = false, threshold_cycling = false;
     bool early_term = false;
+    bool compare = false, truth_zero_based = true;  // -g FILE, -z (main.cpp:678-684)
+    std::string ground_truth;
     int max_colors = 8, et_type = 0;
     GraphElem gen_nv = 0;
     GraphWeight random_edge_pct = 0.0, et_delta = 1.0, threshold = 1.0E-06;
@@ -74,8 +76,8 @@ Options parse(int argc, char** argv)
             case 't': o.early_term = true; o.et_type = std::atoi(optarg); break;
             case 'a': o.et_delta = std::atof(optarg); break;
             case 'i': o.threshold_cycling = true; break;
-            case 'g':
-            case 'z': usage_error("ground-truth comparison (-g/-z) is not part of this build");
+            case 'g': o.ground_truth = optarg; o.compare = true; break;
+            case 'z': o.truth_zero_based = false; break;
             case 'p': o.one_phase = true; break;
             case 'n': o.generate = true; o.gen_nv = std::atol(optarg); break;
             case 'e': o.random_edge_pct = std::atof(optarg); break;
@@ -92,6 +94,7 @@ Options parse(int argc, char** argv)
     if (o.early_term && (o.et_type == 2 || o.et_type == 4) && (o.et_delta > 1.0 || o.et_delta < 0.0))
         usage_error("early-term-alpha must be between 0 and 1: -t 2 -a 0.5");
     if (!o.generate && o.workload.empty() && o.input.empty()) usage_error("Must specify a binary file name with -f");
+    if (o.compare && o.ground_truth.empty()) usage_error("Must specify a ground-truth file name with -g");
     if (o.coloring && o.vertex_order)
         usage_error("Cannot turn on both coloring (-c) and vertex ordering (-d). Re-run using one of the options.");
     if (o.one_phase && o.threshold_cycling)
@@ -151,6 +154,13 @@ void accumulate_communities(const CommunityVector& cvect, int phase, std::vector
 int main(int argc, char** argv)
 {
     const Options o = parse(argc, argv);
+    // main.cpp:119-137: the ground truth is read before the graph
+    std::vector<GraphElem> commGroundTruth;
+    if (o.compare) {
+        const double t1 = now();
+        loadGroundTruthFile(commGroundTruth, o.ground_truth, o.truth_zero_based);
+        std::cout << "Time taken to load ground truth file (" << o.ground_truth << "): " << (now() - t1) << std::endl;
+    }
     const double td0 = now();
     DistGraph* dg = make_graph(o);
     dg->printStats();
@@ -166,7 +176,8 @@ int main(int argc, char** argv)
 
     const GraphElem nv0 = dg->getTotalNumVertices();
     std::vector<GraphElem> commAll;
-    if (o.output_communities) commAll.assign(nv0, -1);
+    const bool track = o.output_communities || o.compare;
+    if (track) commAll.assign(nv0, -1);
     std::vector<GraphElem> ssizes, rsizes, svdata, rvdata;
     size_t ssz = 0, rsz = 0;
     CommunityVector cvect;
@@ -240,7 +251,7 @@ int main(int argc, char** argv)
         ctime += t0 - t1;
 
         if ((currMod - prevMod) > threshold) {
-            if (o.output_communities) accumulate_communities(cvect, phase, commAll);
+            if (track) accumulate_communities(cvect, phase, commAll);
             if (!o.one_phase) {
                 const double t3 = now();
                 distbuildNextLevelGraph(1, 0, dg, ssz, rsz, ssizes, rsizes, svdata, rvdata, cvect);
@@ -291,6 +302,13 @@ int main(int argc, char** argv)
         if (!f) usage_error("Error creating community file: " + out);
         for (GraphElem i = 0; i < nv0; i++) f << commAll[i] << "\n";
         std::cout << "Saved community information (" << commAll.size() << " vertices)  on file: " << out << std::endl;
+    }
+    // main.cpp:574-580
+    if (o.compare) {
+        if (commGroundTruth.size() != commAll.size())
+            usage_error("ground truth holds " + std::to_string(commGroundTruth.size()) + " vertices, the graph " +
+                        std::to_string(commAll.size()));
+        compare_communities(commGroundTruth, commAll);
     }
     delete dg;
     return 0;

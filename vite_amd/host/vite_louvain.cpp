@@ -11,6 +11,8 @@
 #include <iostream>
 #include <map>
 #include <random>
+#include <sstream>
+#include <algorithm>
 
 #include "../../include/vite_amd.h"
 
@@ -338,6 +340,72 @@ void writeGraph(const DistGraph& dg, const std::string& fileName)
     f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
     f.write(reinterpret_cast<const char*>(g.edgeListIndexes.data()), sizeof(GraphElem) * (hdr[0] + 1));
     f.write(reinterpret_cast<const char*>(g.edgeList.data()), sizeof(Edge) * hdr[1]);
+}
+
+// ---- ground truth ---------------------------------------------------------------------------
+
+void loadGroundTruthFile(std::vector<GraphElem>& commGroundTruth, std::string const& groundTruthFileName,
+                         bool isGroundTruthZeroBased)
+{
+    std::ifstream f(groundTruthFileName);
+    if (!f) {
+        std::cerr << "Error opening ground truth file: " << groundTruthFileName << std::endl;
+        std::exit(EXIT_FAILURE);
+    }
+    std::string line;
+    while (std::getline(f, line)) {
+        // as louvain.cpp:3419-3428: the second field of each line, in line order; a line
+        // without it still adds an entry (-1, or -2 from a one-based file)
+        GraphElem v = -1, c = -1;
+        std::istringstream iss(line);
+        iss >> v >> c;
+        if (!isGroundTruthZeroBased) c--;
+        commGroundTruth.push_back(c);
+    }
+    std::cout << "Loaded ground-truth file: " << groundTruthFileName << ", containing community information for "
+              << commGroundTruth.size() << " vertices." << std::endl;
+}
+
+void compare_communities(std::vector<GraphElem> const& C1, std::vector<GraphElem> const& C2)
+{
+    if (C1.empty() || C1.size() != C2.size()) fatal("compare_communities", "partitions differ in size or are empty");
+    vt_compare_stats st;
+    check(vt_compare_communities(vite_amd_context(), (int64_t)C1.size(), C1.data(), C2.data(), &st),
+          "compare_communities");
+    // the report of compare.cpp:274-293
+    std::cout << "Number of unique communities in C1 = " << st.nc_truth << ", and C2 = " << st.nc_found << std::endl;
+    std::cout << "*******************************************" << std::endl;
+    std::cout << "Communities comparison statistics:" << std::endl;
+    std::cout << "*******************************************" << std::endl;
+    std::cout << "|C1| (truth)       : " << st.n << std::endl;
+    std::cout << "#communities in C1 : " << st.nc_truth << std::endl;
+    std::cout << "|C2| (output)      : " << st.n << std::endl;
+    std::cout << "#communities in C2 : " << st.nc_found << std::endl;
+    std::cout << "-------------------------------------------" << std::endl;
+    std::cout << "Same-Same (True positive)  : " << st.same_same << std::endl;
+    std::cout << "Same-Diff (False negative) : " << st.same_diff << std::endl;
+    std::cout << "Diff-Same (False positive) : " << st.diff_same << std::endl;
+    std::cout << "-------------------------------------------" << std::endl;
+    std::cout << "Precision :  " << st.precision << " (" << (st.precision * 100) << ")" << std::endl;
+    std::cout << "Recall    :  " << st.recall << " (" << (st.recall * 100) << ")" << std::endl;
+    std::cout << "F-score   :  " << st.f_score << std::endl;
+    std::cout << "-------------------------------------------" << std::endl;
+    std::cout << "Gini coefficient, C1  :  " << st.gini_truth << std::endl;
+    std::cout << "Gini coefficient, C2  :  " << st.gini_found << std::endl;
+    std::cout << "*******************************************" << std::endl;
+}
+
+// compute_gini_coeff (compare.cpp:310-337): sorts colorSize in place, then the one-thread
+// left folds and the closing expression.
+double compute_gini_coeff(GraphElem* colorSize, int numColors)
+{
+    std::sort(colorSize, colorSize + numColors);
+    double numFunc = 0.0, denFunc = 0.0;
+    for (GraphElem i = 0; i < numColors; i++) {
+        numFunc = numFunc + (double)((i + 1) * colorSize[i]);
+        denFunc = denFunc + (double)colorSize[i];
+    }
+    return ((2 * numFunc) / (numColors * denFunc)) - ((double)(numColors + 1) / (double)numColors);
 }
 
 extern "C" int vt_gen_vite_rgg(vt_ctx* ctx, int64_t n, uint32_t x0, vt_graph** out);

@@ -91,6 +91,15 @@ void generateInMemGraph(int rank, int nprocs, DistGraph*& dg, GraphElem nv, Grap
                         std::string fileOut);
 void writeGraph(const DistGraph& dg, const std::string& fileName);
 
+// ---- ground truth: louvain.hpp:201-203, compare.hpp:56-57 -----------------------------
+// LFR-style "vertex community" lines; community ids one-based unless zero-based is set.
+void loadGroundTruthFile(std::vector<GraphElem>& commGroundTruth, std::string const& groundTruthFileName,
+                         bool isGroundTruthZeroBased);
+// Prints the reference's comparison report for C1 (truth) against C2 (computed); the pair
+// counts and scores come from vt_compare_communities on the device.
+void compare_communities(std::vector<GraphElem> const& C1, std::vector<GraphElem> const& C2);
+double compute_gini_coeff(GraphElem* colorSize, int numColors);
+
 // The workload generators (seeded RGG with extra edges, Barabasi-Albert) on the device.
 DistGraph* generateSeededRGG(unsigned long long seed, GraphElem n, GraphWeight extraPercent);
 DistGraph* generateSeededBA(unsigned long long seed, GraphElem n, GraphElem m);
