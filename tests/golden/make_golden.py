@@ -180,7 +180,7 @@ def main():
         comm = np.loadtxt(path + ".communities", dtype=np.int64, ndmin=1)
         os.remove(path + ".communities")
         out["ground_truth"].append({"graph": name, "truth": kind, "one_based": one_based, "flags": flags,
-                                    "communities": comm.tolist(), "stats": parse_stats(text)})
+                                    "communities": " ".join(map(str, comm.tolist())), "stats": parse_stats(text)})
     for n in [256, 1000, 4096]:
         g = po.ref_gen_rgg(n)
         out["vite_rgg"].append({"n": n, "nv": g.nv, "ne": g.ne, "csr_sha256": csr_sha(g)})

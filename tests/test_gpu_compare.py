@@ -41,7 +41,7 @@ def _truth_vector(kind, nv):
 def test_compare_matches_reference(ctx, idx):
     case = GOLDEN["ground_truth"][idx]
     nv = GOLDEN["graphs"][case["graph"]]["nv"]
-    st = ctx.compare_communities(_truth_vector(case["truth"], nv), np.array(case["communities"], dtype=np.int64))
+    st = ctx.compare_communities(_truth_vector(case["truth"], nv), np.array(case["communities"].split(), dtype=np.int64))
     for k in INT_KEYS:
         assert st[k] == case["stats"][k], k
     for k in FLOAT_KEYS:

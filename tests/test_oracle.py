@@ -372,7 +372,7 @@ def test_golden_ground_truth_compare(golden_graphs, idx):
     case = GOLDEN["ground_truth"][idx]
     g = golden_graphs[case["graph"]]
     truth = _truth_vector(case["truth"], g.nv)
-    st = po.compare(truth, np.array(case["communities"], dtype=np.int64))
+    st = po.compare(truth, np.array(case["communities"].split(), dtype=np.int64))
     assert _stats_as_golden(st) == case["stats"]
 
 
