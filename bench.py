@@ -17,6 +17,7 @@ state (oracle/_ref, OpenMP threads), with the GPU targets checked against it.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
     python bench.py --summarize-profile DIR     (condense rocprofv3 CSV output)
+    python bench.py --hbm-table KT FETCH WRITE  (per-kernel HBM GB/s from three rocprofv3 runs)
 """
 import argparse
 import json
@@ -106,6 +107,51 @@ def summarize_profile(d):
             for k, cs in agg.items():
                 for c, v in cs.items():
                     lines.append(f"{k} | {c} | {len(v)} | {sum(v) / len(v):.6g}")
+    return "\n".join(lines) + "\n"
+
+
+def _short_kernel(name):
+    name = name.replace("(anonymous namespace)::", "")
+    return (name.split("(")[0] if not name.startswith("void rocprim") else name[:60])[:90]
+
+
+def hbm_table(kt_dir, fetch_dir, write_dir, top=30):
+    """Achieved HBM rate of every kernel of one program: dispatches and mean duration from a
+    rocprofv3 --kernel-trace --stats run, HBM bytes per dispatch from FETCH_SIZE x FETCH_SCALE
+    + WRITE_SIZE x WRITE_SCALE (KiB, mean per dispatch) from two --pmc runs of the same
+    program, joined on the full kernel name; GB/s = bytes / mean duration, against 8 TB/s."""
+    import collections
+    import csv
+    import glob
+
+    def csvs(d, suffix):
+        return sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+
+    stats = {}
+    for f in csvs(kt_dir, "kernel_stats.csv"):
+        for row in csv.DictReader(open(f)):
+            stats[row["Name"]] = (int(row["Calls"]), float(row["AverageNs"]))
+
+    def counter(d, cname):
+        agg = collections.defaultdict(list)
+        for f in csvs(d, "counter_collection.csv"):
+            for row in csv.DictReader(open(f)):
+                if row.get("Counter_Name") == cname:
+                    agg[row["Kernel_Name"]].append(float(row["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in agg.items()}
+
+    fetch, write = counter(fetch_dir, "FETCH_SIZE"), counter(write_dir, "WRITE_SIZE")
+    rows = []
+    for name, (calls, ns) in stats.items():
+        if name in fetch and name in write and ns > 0:
+            b = 1024.0 * (fetch[name] * FETCH_SCALE + write[name] * WRITE_SCALE)
+            rows.append((calls * ns, name, calls, ns, b))
+    rows.sort(reverse=True)
+    lines = ["## kernel | calls | mean ms | HBM MB per dispatch | GB/s | % of 8 TB/s"]
+    for _, name, calls, ns, b in rows[:top]:
+        gbs = b / ns
+        lines.append(f"{_short_kernel(name)} | {calls} | {ns / 1e6:.4f} | {b / 1e6:.2f} | {gbs:.1f} | "
+                     f"{100 * gbs / HBM_PEAK_GBS:.1f}")
     return "\n".join(lines) + "\n"
 
 
@@ -320,6 +366,9 @@ def measured_traffic(kernel, workload_tag, symbol):
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--summarize-profile":
         sys.stdout.write(summarize_profile(sys.argv[2]))
+        return
+    if len(sys.argv) == 5 and sys.argv[1] == "--hbm-table":
+        sys.stdout.write(hbm_table(*sys.argv[2:5]))
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
