@@ -614,6 +614,9 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                 if (a.frozen) a.frozen[v] = 0.0;
             }
         } else {
+            // the loads of this vertex have been issued: its LDS work and argmax take the
+            // SIMD ahead of waves still waiting on memory, so it reaches its next loads sooner
+            __builtin_amdgcn_s_setprio(1);
             const double ccDeg = a.cdeg[cc];
             const int64_t ccSize = a.csize[cc];
             if (lane == 0) *ecc_cell = 0.0;
@@ -674,6 +677,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                 if (a.frozen) a.frozen[v] = ecc;
             }
             wave_sync();
+            __builtin_amdgcn_s_setprio(0);
         }
         if (has_next && h.deg <= 0) row_issue_gathers(a, hn.v, rn);
         h = hn;
