@@ -40,6 +40,29 @@ def test_init_degrees(ctx, name):
     assert abs(c_tree - c_o) <= 1e-12 * abs(c_o)
 
 
+def test_long_rows_degree_fold(ctx):
+    """Rows of degree 2048 and more are folded by a wave each, the rest by a thread each:
+    every vDegree is still the row's left fold from 0.0 in CSR order."""
+    rng = np.random.default_rng(17)
+    n = 9000
+    pairs = [(0, int(v)) for v in rng.choice(np.arange(1, n), 7000, replace=False)]
+    pairs += [(1, int(v)) for v in rng.choice(np.arange(2, n), 2048, replace=False)]
+    pairs += [(2, int(v)) for v in rng.choice(np.arange(3, n), 2047, replace=False)]
+    a, b = rng.integers(3, n, 20000), rng.integers(3, n, 20000)
+    pairs += [(int(x), int(y)) for x, y in zip(a, b) if x != y]
+    pairs = np.unique(np.sort(np.array(pairs), axis=1), axis=0)
+    g = po.from_edge_list(n, pairs, rng.uniform(0.01, 1.0, len(pairs)))
+    assert np.diff(g.offs).max() >= 7000
+    dg = ctx.upload(vite_amd.CSR(g.offs, g.edges))
+    vdeg_o, c_o = po.init_degrees(g)
+    vdeg, c = dg.init_degrees(sequential=True)
+    assert np.array_equal(vdeg.view(np.uint64), vdeg_o.view(np.uint64))
+    assert c == c_o
+    q_o, cv_o, it_o = po.louvain_phase(g)
+    q, cv, it = dg.louvain_phase(sequential=True)
+    assert (q, it) == (q_o, it_o) and np.array_equal(cv, cv_o)
+
+
 @pytest.mark.parametrize("name", graphs.ALL)
 @pytest.mark.parametrize("seed", [None, 1, 2])
 def test_sweep_replay(ctx, name, seed):

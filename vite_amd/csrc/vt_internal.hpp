@@ -189,6 +189,15 @@ __device__ __forceinline__ double fold_rest(double acc, mask_t rest, const doubl
     return acc;
 }
 
+// Lane j's double, read as a wave-uniform value (two 32-bit lane reads).
+__device__ __forceinline__ double readlane_f64(double x, int j)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // ---- device primitives (vt_prims.hip) -------------------------------------------------
 // Deterministic fixed-order sums: every block reduces a fixed contiguous chunk with a
 // fixed tree; block partials are reduced by one block in index order.

@@ -1218,15 +1218,19 @@ __global__ void k_hbm_table_init(int32_t* keys, double* vals, int64_t n)
 // vDegree: left fold of each row's weights from 0.0 in CSR order (distSumVertexDegree,
 // louvain.cpp:2161-2186), one thread per row.  Eight loads are issued ahead of the dependent
 // adds, so a long row (coarse levels, hubs) streams at memory rate instead of one load
-// latency per edge.
+// latency per edge.  Rows of degree >= long_min are left to k_row_fold_wave.
 constexpr int kFoldRows = 256;
+// Rows at least this long get a wave each when a graph has them (a hub of degree 36 K
+// folded by one thread held the whole launch for 3 ms).
+constexpr int64_t kFoldLongRow = 2048;
 
 __global__ void __launch_bounds__(kFoldRows) k_row_fold(const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
-                                                        int64_t nv, double* __restrict__ vdeg)
+                                                        int64_t nv, double* __restrict__ vdeg, int64_t long_min)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t e = offs[i + 1];
         int64_t k = offs[i];
+        if (e - k >= long_min) continue;
         double acc = 0.0;
         for (; k + 8 <= e; k += 8) {
             double w[8];
@@ -1240,29 +1244,26 @@ __global__ void __launch_bounds__(kFoldRows) k_row_fold(const int64_t* __restric
     }
 }
 
-// Long rows (the coarse levels): one wave per row loads 64 weights at a time with coalesced
-// reads (the next 64 in flight while these are added), stages them in LDS, and lane 0 adds
-// them in order -- the same left fold as k_row_fold.
+// Long rows (the coarse levels, hubs): one wave per row loads 64 weights at a time with
+// coalesced reads (the next 64 in flight while these are added), and every lane adds them in
+// lane order, each read as a wave-uniform value -- the same left fold as k_row_fold.  Lanes
+// past the row's end hold -0.0, which leaves any sum unchanged.  Rows shorter than long_min
+// are left to k_row_fold.
 constexpr int kFoldWaveRows = 4;
 __global__ void __launch_bounds__(64 * kFoldWaveRows) k_row_fold_wave(const int64_t* __restrict__ offs,
                                                                      const DEdge* __restrict__ edges, int64_t nv,
-                                                                     double* __restrict__ vdeg)
+                                                                     double* __restrict__ vdeg, int64_t long_min)
 {
-    __shared__ double st[kFoldWaveRows][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int64_t i = (int64_t)blockIdx.x * kFoldWaveRows + w; i < nv; i += (int64_t)gridDim.x * kFoldWaveRows) {
         const int64_t e0 = offs[i], e1 = offs[i + 1];
+        if (e1 - e0 < long_min) continue;  // wave-uniform: i is the same in every lane
         double acc = 0.0;
-        double x = (e0 + lane < e1) ? edges[e0 + lane].weight : 0.0;
+        double x = (e0 + lane < e1) ? edges[e0 + lane].weight : -0.0;
         for (int64_t base = e0; base < e1; base += 64) {
-            const double xn = (base + 64 + lane < e1) ? edges[base + 64 + lane].weight : 0.0;
-            st[w][lane] = x;
-            wave_sync();
-            if (lane == 0) {
-                const int n = (int)(e1 - base < 64 ? e1 - base : 64);
-                for (int u = 0; u < n; u++) acc = acc + st[w][u];
-            }
-            wave_sync();
+            const double xn = (base + 64 + lane < e1) ? edges[base + 64 + lane].weight : -0.0;
+#pragma unroll
+            for (int u = 0; u < 64; u++) acc = acc + readlane_f64(x, u);
             x = xn;
         }
         if (lane == 0) vdeg[i] = acc;
@@ -1698,12 +1699,18 @@ struct Phase {
             curr32 = DBuf<int32_t>(ctx, (size_t)nv);
             VT_LAUNCH(ctx, "split_fold", k_split_fold, grid_for(nv, kSfRows, 65536), 256, 0, g->offs, g->edges, nv,
                       tail32.p, wgt.p, vdeg.p);
-        } else if (nv && nv < 65536 && g->ne > 48 * nv)  // few, long rows: a thread per row would leave the GPU idle
+        } else if (nv && nv < 65536 && g->ne > 48 * nv) {  // few, long rows: a thread per row would leave the GPU idle
             VT_LAUNCH(ctx, "row_fold", k_row_fold_wave, grid_for(nv, kFoldWaveRows, 65536), 64 * kFoldWaveRows, 0, g->offs,
-                      g->edges, nv, vdeg.p);
-        else if (nv)
+                      g->edges, nv, vdeg.p, (int64_t)0);
+        } else if (nv) {
+            // a thread per row, and a wave per row for the rows of kFoldLongRow or more
+            const bool split = g->max_degree >= kFoldLongRow;
             VT_LAUNCH(ctx, "row_fold", k_row_fold, grid_for(nv, kFoldRows, 16384), kFoldRows, 0, g->offs, g->edges, nv,
-                      vdeg.p);
+                      vdeg.p, split ? kFoldLongRow : INT64_MAX);
+            if (split)
+                VT_LAUNCH(ctx, "row_fold_long", k_row_fold_wave, grid_for(nv, kFoldWaveRows, 16384), 64 * kFoldWaveRows, 0,
+                          g->offs, g->edges, nv, vdeg.p, kFoldLongRow);
+        }
         VT_LAUNCH(ctx, "init", k_init, grid_for(nv, 256, 8192), 256, 0, nv, (const double*)vdeg.p, cdeg.p, csize.p, past.p,
                   curr.p, target.p, with_et ? active.p : nullptr, with_et ? frozen.p : nullptr,
                   with_et ? pa.p : nullptr, with_et ? pb.p : nullptr);

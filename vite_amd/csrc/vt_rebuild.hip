@@ -118,14 +118,6 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ double readlane_f64(double x, int j)
-{
-    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, j);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // Positions of community c's concatenated row -> CSR indices (moff: each member's start in
 // its community's concatenated row; mrow: its row start in the CSR).  Up to 64 members,
 // lane i holds member i's two starts and lanes find their member with shuffles; larger
