@@ -40,6 +40,31 @@ def test_init_degrees(ctx, name):
     assert abs(c_tree - c_o) <= 1e-12 * abs(c_o)
 
 
+def test_long_event_runs(ctx):
+    """A community that thousands of vertices join in one sweep (the centre of a star with
+    weighted leaves, plus a ring among the leaves): its run of move events is folded by a
+    wave, the short runs by a thread each; the per-community degree and size changes stay
+    bit-identical to the oracle's in-order folds."""
+    rng = np.random.default_rng(23)
+    n = 5001
+    pairs = [(0, v) for v in range(1, n)] + [(v, v + 1) for v in range(1, n - 1)]
+    w = np.concatenate([rng.uniform(0.5, 1.0, n - 1), rng.uniform(0.01, 0.1, n - 2)])
+    g = po.from_edge_list(n, np.array(pairs), w)
+    dg = ctx.upload(vite_amd.CSR(g.offs, g.edges))
+    vdeg, c = po.init_degrees(g)
+    st = (np.arange(n), np.ones(n, np.int64), vdeg.copy(), vdeg, c)
+    t_o, cw_o, ds_o, dd_o = po.sweep(g, *st)
+    assert np.count_nonzero(t_o == 0) > 2 * 1024  # the long run this test is about
+    t, cw, ds, dd = dg.sweep(*st)
+    assert np.array_equal(t, t_o)
+    assert np.array_equal(cw.view(np.uint64), cw_o.view(np.uint64))
+    assert np.array_equal(ds, ds_o)
+    assert np.array_equal(dd.view(np.uint64), dd_o.view(np.uint64))
+    q_o, cv_o, it_o = po.louvain_phase(g)
+    q, cv, it = dg.louvain_phase(sequential=True)
+    assert (q, it) == (q_o, it_o) and np.array_equal(cv, cv_o)
+
+
 def test_long_rows_degree_fold(ctx):
     """Rows of degree 2048 and more are folded by a wave each, the rest by a thread each:
     every vDegree is still the row's left fold from 0.0 in CSR order."""

@@ -1394,6 +1394,11 @@ __global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t
 // head of each run folds localCupdate exactly as one thread's sequence of `omp atomic`
 // updates does (+= k_i on arrival, -= k_i on departure), then localCinfo += localCupdate
 // (louvain.cpp:2534-2547).
+// Runs of at least this many events (a community that many movers join or leave: hubs on
+// skewed graphs) are folded by a wave each in k_fold_events_long; a thread folding one such
+// run held a whole launch for 7 ms on BA n = 2^22.
+constexpr int64_t kFoldLongRun = 1024;
+
 __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* __restrict__ delta,
                               const int8_t* __restrict__ dir, int64_t n, const int64_t* __restrict__ d_movers,
                               double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none)
@@ -1402,6 +1407,7 @@ __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* _
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
         if (c == none || (j > 0 && keys[j - 1] == c)) continue;
+        if (j + kFoldLongRun - 1 < n && keys[j + kFoldLongRun - 1] == c) continue;  // a long run
         double acc = 0.0;
         int64_t cnt = 0;
         int64_t k = j;
@@ -1420,6 +1426,53 @@ __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* _
         }
         cdeg[c] = cdeg[c] + acc;
         csize[c] += cnt;
+    }
+}
+
+// The runs k_fold_events leaves (kFoldLongRun events or more): each wave scans 64-event
+// windows for their starts; for each, the wave loads the run 64 events at a time and adds
+// the deltas in event order as wave-uniform lane reads (lanes past the run's end hold -0.0,
+// which leaves the sum unchanged) -- the same left fold from 0.0 as k_fold_events.  The
+// member-count changes are integers, summed in any order.
+__global__ void __launch_bounds__(256) k_fold_events_long(const uint32_t* __restrict__ keys,
+                                                          const double* __restrict__ delta,
+                                                          const int8_t* __restrict__ dir, int64_t n,
+                                                          const int64_t* __restrict__ d_movers,
+                                                          double* __restrict__ cdeg, int64_t* __restrict__ csize,
+                                                          uint32_t none)
+{
+    if (d_movers) n = 2 * d_movers[0];
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64; base < n; base += waves * 64) {
+        const int64_t j = base + lane;
+        bool start = false;
+        if (j < n) {
+            const uint32_t c = keys[j];
+            start = c != none && (j == 0 || keys[j - 1] != c) && j + kFoldLongRun - 1 < n &&
+                    keys[j + kFoldLongRun - 1] == c;
+        }
+        for (unsigned long long sm = __ballot(start); sm; sm &= sm - 1) {
+            const int64_t j0 = base + __builtin_ctzll(sm);
+            const uint32_t c = keys[j0];
+            double acc = 0.0;
+            int64_t cnt = 0;
+            for (int64_t k = j0;; k += 64) {
+                const int64_t idx = k + lane;
+                const bool in = idx < n && keys[idx] == c;
+                const double d = in ? delta[idx] : -0.0;
+                int v = in ? (int)dir[idx] : 0;
+#pragma unroll
+                for (int u = 0; u < 64; u++) acc = acc + readlane_f64(d, u);
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                cnt += v;
+                if (__ballot(in) != ~0ull) break;  // the run ends inside this window
+            }
+            if (lane == 0) {
+                cdeg[c] = cdeg[c] + acc;
+                csize[c] += cnt;
+            }
+        }
     }
 }
 
@@ -1973,6 +2026,9 @@ struct Phase {
                   (const int64_t*)target.p, (const double*)vdeg.p, none, delta, dir);
         VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, (const double*)delta,
                   (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none);
+        if (ne >= kFoldLongRun)
+            VT_LAUNCH(ctx, "fold_events_long", k_fold_events_long, grid_for(ne, 256, 4096), 256, 0, ks,
+                      (const double*)delta, (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none);
     }
 
     double modularity(int flags)
