@@ -1376,9 +1376,10 @@ __global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, cons
 __global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
                                const int64_t* __restrict__ d_movers, const int64_t* __restrict__ target,
                                const double* __restrict__ vdeg, uint32_t none, double* __restrict__ delta,
-                               int8_t* __restrict__ dir)
+                               int8_t* __restrict__ dir, unsigned long long* __restrict__ long_runs)
 {
     if (d_movers) n = 2 * d_movers[0];  // the count the device computed (a replayed graph's n is fixed)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *long_runs = 0;  // k_fold_events marks long runs
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
         if (c == none) continue;
@@ -1401,13 +1402,17 @@ constexpr int64_t kFoldLongRun = 1024;
 
 __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* __restrict__ delta,
                               const int8_t* __restrict__ dir, int64_t n, const int64_t* __restrict__ d_movers,
-                              double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none)
+                              double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none,
+                              unsigned long long* __restrict__ long_runs)
 {
     if (d_movers) n = 2 * d_movers[0];
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t c = keys[j];
         if (c == none || (j > 0 && keys[j - 1] == c)) continue;
-        if (j + kFoldLongRun - 1 < n && keys[j + kFoldLongRun - 1] == c) continue;  // a long run
+        if (j + kFoldLongRun - 1 < n && keys[j + kFoldLongRun - 1] == c) {  // a long run
+            *long_runs = 1;
+            continue;
+        }
         double acc = 0.0;
         int64_t cnt = 0;
         int64_t k = j;
@@ -1439,8 +1444,9 @@ __global__ void __launch_bounds__(256) k_fold_events_long(const uint32_t* __rest
                                                           const int8_t* __restrict__ dir, int64_t n,
                                                           const int64_t* __restrict__ d_movers,
                                                           double* __restrict__ cdeg, int64_t* __restrict__ csize,
-                                                          uint32_t none)
+                                                          uint32_t none, const unsigned long long* __restrict__ long_runs)
 {
+    if (*long_runs == 0) return;  // k_fold_events found none (the common case)
     if (d_movers) n = 2 * d_movers[0];
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * 4;
@@ -2023,12 +2029,13 @@ struct Phase {
                      double* dst_cdeg, int64_t* dst_csize, const int64_t* d_movers = nullptr)
     {
         VT_LAUNCH(ctx, "event_deltas", k_event_deltas, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, d_movers,
-                  (const int64_t*)target.p, (const double*)vdeg.p, none, delta, dir);
+                  (const int64_t*)target.p, (const double*)vdeg.p, none, delta, dir, ucount.p + 1);
         VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, (const double*)delta,
-                  (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none);
+                  (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none, ucount.p + 1);
         if (ne >= kFoldLongRun)
             VT_LAUNCH(ctx, "fold_events_long", k_fold_events_long, grid_for(ne, 256, 4096), 256, 0, ks,
-                      (const double*)delta, (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none);
+                      (const double*)delta, (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none,
+                      (const unsigned long long*)(ucount.p + 1));
     }
 
     double modularity(int flags)
