@@ -5,8 +5,9 @@
 // this round carries a color >= nextColor, which the reference's skip test ignores, so the
 // round reads only round-start state, coloring.cpp:173-181).  A vertex that is the strict
 // or tied minimum / maximum for some hash takes a color from the remaining min/max slots
-// (coloring.cpp:220-242).  One lane per vertex; rounds repeat until the reference's stop
-// rule fires (single round, >= target_pct colored, or no progress).
+// (coloring.cpp:220-242).  One lane per vertex, or a wave per vertex for rows of degree
+// kColorLongRow and more; rounds repeat until the reference's stop rule fires (single
+// round, >= target_pct colored, or no progress).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
