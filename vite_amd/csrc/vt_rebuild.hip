@@ -302,6 +302,20 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
 
 // Every entry of the communities that overflowed the table, as ((C1 << bits) | C2, weight),
 // at xoff[C1] + its position in C1's concatenated row.
+// Overflowing communities of more than this many entries (a giant community on a skewed
+// graph) are emitted by every wave of k_ovf_emit_big instead of one wave each: emitting
+// needs no order, and one wave walking such a community held the launch for 4 ms on BA
+// n = 2^22.
+constexpr int64_t kOvfBig = 1 << 16;
+
+// The ids of those communities, in any order (their entries' places do not depend on it).
+__global__ void k_ovf_big_list(int64_t nc, const int64_t* __restrict__ xcnt, unsigned long long* __restrict__ count,
+                               int64_t* __restrict__ list)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+        if (xcnt[c] > kOvfBig) list[atomicAdd(count, 1ull)] = c;
+}
+
 __global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit(
     int64_t nc, const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow,
     const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
@@ -310,11 +324,40 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit(
 {
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
-        if (xcnt[c] == 0) continue;
+        if (xcnt[c] == 0 || xcnt[c] > kOvfBig) continue;  // k_ovf_emit_big takes the giant ones
         const MemberRows mr(cstart, moff, mrow, c, lane);
         const int64_t o = xoff[c] - mr.s0;
         for (int64_t base = mr.s0; base < mr.s1; base += 64) {
             const int64_t p = base + lane;
+            const int64_t k = mr.csr_index(p, moff, mrow);
+            if (p < mr.s1) {
+                const DEdge e = edges[k];
+                keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
+                vals[o + p] = e.weight;
+            }
+        }
+    }
+}
+
+// The giant overflowing communities: for each listed community, the whole grid deals its
+// 64-entry chunks to the waves; each lane finds its entry's CSR index by a search of the
+// member offsets, as k_ovf_emit does for communities of more than 64 members.
+__global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit_big(
+    const unsigned long long* __restrict__ count, const int64_t* __restrict__ list, const int64_t* __restrict__ cstart,
+    const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow, const DEdge* __restrict__ edges,
+    const uint32_t* __restrict__ newid, const int64_t* __restrict__ xoff, int bits, unsigned long long* __restrict__ keys,
+    double* __restrict__ vals)
+{
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t wave = (int64_t)blockIdx.x * kCfWaves + w, waves = (int64_t)gridDim.x * kCfWaves;
+    const int64_t nbig = (int64_t)*count;
+    for (int64_t i = 0; i < nbig; i++) {
+        const int64_t c = list[i];
+        const MemberRows mr(cstart, moff, mrow, c, lane);
+        const int64_t o = xoff[c] - mr.s0;
+        const int64_t nchunks = (mr.s1 - mr.s0 + 63) / 64;
+        for (int64_t ch = wave; ch < nchunks; ch += waves) {
+            const int64_t p = mr.s0 + ch * 64 + lane;
             const int64_t k = mr.csr_index(p, moff, mrow);
             if (p < mr.s1) {
                 const DEdge e = edges[k];
@@ -426,6 +469,14 @@ static void overflow_rows(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid,
     DBuf<double> v0(ctx, x1), v1(ctx, x1);
     VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, moff, mrow,
               (const DEdge*)g->edges, newid, xcnt, xoff, bits, k0.p, v0.p);
+    if (nx > kOvfBig) {  // a community that big may exist
+        DBuf<unsigned long long> nbig(ctx, 1);
+        DBuf<int64_t> big(ctx, (size_t)std::max<int64_t>(nx / kOvfBig, 1));
+        dzero(ctx, nbig.p, 8);
+        VT_LAUNCH(ctx, "rb_ovf_big_list", k_ovf_big_list, grid_for(nc, 256, 4096), 256, 0, nc, xcnt, nbig.p, big.p);
+        VT_LAUNCH(ctx, "rb_ovf_emit_big", k_ovf_emit_big, 1024, 64 * kCfWaves, 0, (const unsigned long long*)nbig.p,
+                  (const int64_t*)big.p, cstart, moff, mrow, (const DEdge*)g->edges, newid, xoff, bits, k0.p, v0.p);
+    }
     const unsigned long long* ks = k0.p;
     const double* vs = v0.p;
     if (nx > 1) {
