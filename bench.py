@@ -46,8 +46,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dg, st_state, threads, reps, gpu_target):
-    """The reference's sweep loop on the host, same graph and state."""
+def host_cpus():
+    """The host's CPUs as this process may use them: logical CPUs in the affinity mask, the
+    physical cores they belong to (lscpu's CORE/SOCKET columns), the cgroup CPU quota if
+    one is set, and the thread count used for the CPU baseline: the physical cores in the
+    mask, capped by the quota."""
+    import subprocess
+
+    mask = sorted(os.sched_getaffinity(0))
+    info = {"logical_in_mask": len(mask), "physical_in_mask": None, "cgroup_quota_cpus": None, "model": None}
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        core_of = {}
+        for line in out.splitlines():
+            if line and not line.startswith("#"):
+                cpu, core, sock = (int(x) for x in line.split(",")[:3])
+                core_of[cpu] = (sock, core)
+        info["physical_in_mask"] = len({core_of[c] for c in mask if c in core_of}) or None
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_quota_cpus"] = int(quota) / int(period)
+    except (OSError, ValueError):
+        pass
+    threads = info["physical_in_mask"] or len(mask)
+    if info["cgroup_quota_cpus"]:
+        threads = max(1, min(threads, int(info["cgroup_quota_cpus"])))
+    info["threads_used"] = threads
+    return info
+
+
+def cpu_baseline(dg, st_state, threads, reps, gpu_target, host):
+    """The reference's sweep loop on the host, same graph and state: on every usable core
+    (mean of reps full sweeps) and on one thread (mean of reps sweeps over a prefix of the
+    vertices holding about 1.5e8 entries, a bounded sample of the same work)."""
     from oracle import pyoracle as po
     import numpy as np
 
@@ -57,10 +94,20 @@ def cpu_baseline(dg, st_state, threads, reps, gpu_target):
     if po.ref_available():
         sec, tgt = po.ref_sweep_omp(g, *args, 0, g.nv, threads, reps)
         mism = int(np.count_nonzero(tgt != gpu_target))
+        v1 = int(min(g.nv, max(1, np.searchsorted(g.offs, 150_000_000))))
+        e1 = int(g.offs[v1])
+        sec1, tgt1 = po.ref_sweep_omp(g, *args, 0, v1, 1, reps)
+        mism1 = int(np.count_nonzero(tgt1[:v1] != gpu_target[:v1]))
         return {"value": g.ne / sec, "unit": "edges/s", "cores": threads, "kind": "reference",
                 "sample": f"full sweep of all {g.nv} vertices ({g.ne} directed entries) from the benchmark state, "
-                          f"reference distExecuteLouvainIteration under omp for schedule(guided), mean of {reps}",
-                "seconds_per_sweep": sec, "target_mismatches_vs_gpu": mism}
+                          f"reference distExecuteLouvainIteration (with its localCupdate atomics) under omp for "
+                          f"schedule(guided), {threads} threads = physical cores in the affinity mask"
+                          + (" capped by the cgroup quota" if host.get("cgroup_quota_cpus") else "")
+                          + f", mean of {reps}",
+                "seconds_per_sweep": sec, "target_mismatches_vs_gpu": mism, "host": host,
+                "one_thread": {"value": e1 / sec1, "unit": "edges/s", "seconds": sec1, "reps": reps,
+                               "sample": f"the first {v1} vertices ({e1} entries) of the same sweep",
+                               "target_mismatches_vs_gpu": mism1}}
     # the reference was not built: time the single-threaded oracle on a vertex sample
     nsamp = min(g.nv, 1 << 20)
     order = np.arange(nsamp, dtype=np.int64)
@@ -71,7 +118,69 @@ def cpu_baseline(dg, st_state, threads, reps, gpu_target):
     mism = int(np.count_nonzero(t[:nsamp] != gpu_target[:nsamp]))
     return {"value": e / sec, "unit": "edges/s", "cores": 1, "kind": "port",
             "sample": f"oracle sweep of the first {nsamp} vertices ({e} entries) from the benchmark state",
-            "seconds_per_sweep": sec, "target_mismatches_vs_gpu": mism}
+            "seconds_per_sweep": sec, "target_mismatches_vs_gpu": mism, "host": host}
+
+
+def config1_time_to_q(ctx, args, host):
+    """Time to final Q on the CPU-runnable configuration (RGG n = 65,536, threshold 1e-6, no
+    coloring or early termination), on this host: the reference's own main() (oracle/_ref,
+    1 MPI rank; its "Average time for clustering", main.cpp:522-528) at one thread and on
+    every usable core, against the device's whole run from the resident graph."""
+    import tempfile
+
+    from oracle import pyoracle as po
+
+    n = 65536
+    dg = ctx.gen_rgg(args.seed, n, 0.0)
+    dg.run()  # warm the kernels and the allocator
+    t0 = time.perf_counter()
+    r = dg.run()
+    dev_s = time.perf_counter() - t0
+    out = {"workload": f"RGG n={n} r=sqrt(2 ln n/(pi n)), seed {args.seed}, threshold 1e-6",
+           "device": {"seconds": dev_s, "final_q": r.final_q, "calls": len(r.call_q),
+                      "total_iters": r.total_iters}}
+    if po.ref_available():
+        csr = dg.download()
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "rgg65536.bin")
+            po.write_bin(path, po.Graph(csr.offs, csr.edges))
+            for label, thr in (("one_thread", 1), ("all_cores", host["threads_used"])):
+                secs = []
+                for _ in range(3):
+                    recs, text = po.ref_trace_run(path, [], threads=thr, timeout=600)
+                    m = re.search(r"Average time for clustering \(secs\.\): (\S+)", text)
+                    secs.append(float(m.group(1)))
+                q = float([x for x in recs if x["kind"] != "rebuild"][-1]["q"])
+                out[label] = {"threads": thr, "clustering_seconds": sum(secs) / len(secs), "reps": len(secs),
+                              "final_q": q, "abs_diff_final_q_vs_device": abs(q - r.final_q)}
+            out["speedup_vs_all_cores"] = out["all_cores"]["clustering_seconds"] / dev_s
+    dg.free()
+    return out
+
+
+def iteration_timing(st, V, E, steps=3):
+    """Whole Louvain iterations (sweep, the per-community delta folds that replace
+    localCupdate / distUpdateLocalCinfo, modularity, rotation) continued from the benchmark
+    state, each timed alone.  Algorithmic bytes: 24E + 56V (sweep) + 8V (mover flags) + 64 B
+    per mover (two 32-byte events) + 24V (modularity)."""
+    import numpy as np
+
+    its = []
+    for _ in range(steps):
+        st.sweep()
+        s = st.state()
+        movers = int(np.count_nonzero(s["target"] != s["curr"]))
+        t0 = time.perf_counter()
+        q, done = st.step()
+        sec = time.perf_counter() - t0
+        b = 24.0 * E + 88.0 * V + 64.0 * movers
+        its.append({"ms": 1e3 * sec, "movers": movers, "bytes": b, "hbm_frac": b / sec / 1e9 / HBM_PEAK_GBS, "q": q})
+        if done:
+            break
+    ms = sum(i["ms"] for i in its) / len(its)
+    return {"ms_per_iteration": ms, "edges_per_s": E / (ms * 1e-3),
+            "hbm_frac": sum(i["hbm_frac"] for i in its) / len(its), "iterations": its,
+            "bytes_formula": "24E + 88V + 64 per mover over the iteration's wall time"}
 
 
 def summarize_profile(d):
@@ -378,8 +487,9 @@ def main():
     ap.add_argument("--extra", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--prep", type=int, default=2, help="phase iterations before the timed sweeps")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, host cores)")
-    ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = the physical cores in the affinity mask, capped by the cgroup CPU quota")
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttq", action="store_true", help="skip the time-to-final-Q run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
@@ -443,6 +553,9 @@ def main():
     kavg, kcount, ktot = ctx.kernel_stats("local_move")
     ctx.set_profiling(False)
     state = st.state()
+    per_iter = iteration_timing(st, V, E)
+    log(f"whole iterations: {per_iter['ms_per_iteration']:.3f} ms each "
+        f"({100 * per_iter['hbm_frac']:.1f}% of HBM peak in algorithmic bytes)")
     st.end(want_cvect=False)
 
     # the dominant kernel's own share: the rows of bucket 2 under the tuning in force
@@ -475,6 +588,7 @@ def main():
                      "kernel_rows": Vf, "kernel_entries": Ef,
                      "traffic_source": traffic_src,
                      "traffic_formula": f"FETCH_SIZE x {FETCH_SCALE} + WRITE_SIZE x {WRITE_SCALE} (KiB per dispatch)"},
+        "per_iteration": per_iter,
     }
 
     if not args.no_ttq:
@@ -489,9 +603,10 @@ def main():
             dq = max(abs(a - b) for a, b in zip(r.call_q, ref["q"])) if len(ref["q"]) == len(r.call_q) else None
             out["time_to_final_q"]["reference"] = {
                 "final_q": ref["q"][-1], "abs_diff_final_q": abs(r.final_q - ref["q"][-1]), "max_abs_diff_call_q": dq,
-                "same_iterations": ref["iters"] == r.call_iters, "seconds": ref["seconds"],
-                "source": "the reference's main() -i on the same graph, 1 MPI rank, run in the build container "
-                          f"({ref['threads']} OpenMP threads; not the GPU host)"}
+                "same_iterations": ref["iters"] == r.call_iters, "build_container_seconds": ref["seconds"],
+                "source": "the reference's main() -i on the same graph, 1 MPI rank, run once in the build "
+                          f"container ({ref['threads']} OpenMP threads), NOT on this host; the same-host "
+                          "comparison is time_to_final_q_config1"}
         log(f"time to final Q: {ttq:.2f}s q={r.final_q:.9f} levels={len(r.level_q)} iters={r.total_iters}")
 
     if args.skewed_n > 0:
@@ -502,11 +617,17 @@ def main():
         out["serving"] = serving_form(args)
 
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {threads} threads ...")
-        out["cpu_baseline"] = cpu_baseline(dg, state, threads, args.cpu_reps, state["target"])
+        host = host_cpus()
+        if args.cpu_threads:
+            host["threads_used"] = args.cpu_threads
+        threads = host["threads_used"]
+        log(f"cpu baseline: {threads} threads ({host}) ...")
+        out["cpu_baseline"] = cpu_baseline(dg, state, threads, args.cpu_reps, state["target"], host)
         log(f"cpu baseline: {out['cpu_baseline']['value'] / 1e6:.1f} M edges/s, "
             f"mismatches {out['cpu_baseline']['target_mismatches_vs_gpu']}")
+        if not args.no_ttq:
+            out["time_to_final_q_config1"] = config1_time_to_q(ctx, args, host)
+            log(f"config-1 time to final Q: {out['time_to_final_q_config1']}")
     dg.free()
     ctx.close()
     print(json.dumps(out), flush=True)

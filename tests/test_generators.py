@@ -71,7 +71,7 @@ def test_rgg_extra_edges(seed, n, pct):
         assert 0.0 <= w < 1.0
 
 
-@pytest.mark.parametrize("seed,n,m", [(1, 2, 1), (2, 3000, 4), (3, 20000, 8)])
+@pytest.mark.parametrize("seed,n,m", [(1, 2, 1), (2, 3000, 4), (3, 20000, 8), (4, 20000, 2)])
 def test_ba_structure(seed, n, m):
     g = po.gen_ba(seed, n, m)
     h, t, w = _directed_pairs(g)
@@ -88,5 +88,13 @@ def test_ba_structure(seed, n, m):
         assert np.all(np.diff(row) > 0)
     if n >= 3000:
         deg = np.diff(g.offs)
-        assert deg.max() >= 8 * m                   # preferential attachment: hubs far above m
         assert np.median(deg) <= 2 * m + 1
+        # preferential attachment: the Barabasi-Albert degree law P(k) = 2m(m+1)/(k(k+1)(k+2)),
+        # whose tail count is n m(m+1)/(K(K+1)) vertices of degree >= K, within Poisson noise
+        # plus 5% (uniform attachment has an exponential tail: no vertex of degree 10m at
+        # n = 20000, m = 8, against 222 here)
+        for K in (2 * m, 5 * m, 10 * m):
+            law = n * m * (m + 1) / (K * (K + 1))
+            assert abs(int((deg >= K).sum()) - law) <= 4 * np.sqrt(law) + 0.05 * law, K
+        # the largest hub grows as m sqrt(n) (uniform attachment: about m ln n)
+        assert deg.max() >= 0.5 * m * np.sqrt(n)

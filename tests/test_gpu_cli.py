@@ -6,6 +6,12 @@ main.cpp:380-425 and written as in main.cpp:543-572) and the final modularity co
 Both must equal what the compiled reference wrote for the same file and flags
 (tests/golden/reference_runs.json, "communities").  Sums follow the reference order
 (VITE_AMD_SEQUENTIAL_SUMS=1), so every decision and so the partition is identical.
+
+Each case runs through two drivers: vite_amd's own CLI, and the reference's main.cpp
+compiled unmodified against its own headers and linked against libvite_host.so
+(oracle/_ref/graphClustering_vite_amd, built by oracle/Makefile.ref).  The second is the
+drop-in proof: the reference's driver code, its Graph / DistGraph objects and its call
+sequence, with every Louvain phase, rebuild, coloring and comparison run on the device.
 """
 import hashlib
 import json
@@ -23,6 +29,8 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "vite_amd", "bin", "vite_amd_cluster")
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "graphClustering_vite_amd")
+DRIVERS = {"cli": CLI, "dropin": DROPIN}
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_runs.json")))
 CASES = GOLDEN["communities"]
 
@@ -37,21 +45,32 @@ def _build(spec):
     return graphs.graph(spec["name"])
 
 
+def _driver(name):
+    exe = DRIVERS[name]
+    if not os.path.isfile(exe):
+        pytest.skip(f"{exe} not built (the drop-in driver needs the reference tree at build time)")
+    return exe
+
+
+@pytest.mark.parametrize("driver", sorted(DRIVERS))
 @pytest.mark.parametrize("idx", range(len(CASES)),
                          ids=[f"{c['graph']}:{' '.join(c['flags']) or 'default'}" for c in CASES])
-def test_cli_community_file_matches_reference(tmp_path, idx):
+def test_cli_community_file_matches_reference(tmp_path, idx, driver):
+    exe = _driver(driver)
     case = CASES[idx]
     g = _build(GOLDEN["graphs"][case["graph"]]["spec"])
     path = str(tmp_path / (case["graph"] + ".bin"))
     po.write_bin(path, g)
     env = dict(os.environ, VITE_AMD_SEQUENTIAL_SUMS="1")
-    out = subprocess.run([CLI, "-f", path, "-o"] + case["flags"], capture_output=True, text=True, env=env,
-                         timeout=120)
+    out = subprocess.run([exe, "-f", path, "-o"] + case["flags"], capture_output=True, text=True, env=env,
+                         timeout=120, cwd=str(tmp_path))
     assert out.returncode == 0, out.stderr[-2000:]
     comm = np.loadtxt(path + ".communities", dtype=np.int64, ndmin=1)
     assert len(comm) == case["nv"]
     assert len(np.unique(comm)) == case["count"]
     assert hashlib.sha256(np.ascontiguousarray(comm).tobytes()).hexdigest() == case["sha256"]
+    if driver == "dropin":
+        return  # the reference's driver prints Q only to six digits
     # the last Louvain call's Q; after a one-phase run (-p) the loop has already reset
     # currMod to -1, as the reference's MODS line shows (main.cpp:474-482, 526-528)
     expect = -1.0 if "-p" in case["flags"] else float.fromhex(case["final_q_hex"])
@@ -74,12 +93,14 @@ def _truth_vector(kind, nv):
     return v // 37 if kind == "blocks" else 3 * (v // 50) + 1
 
 
+@pytest.mark.parametrize("driver", sorted(DRIVERS))
 @pytest.mark.parametrize("idx", range(len(TRUTH)),
                          ids=[f"{c['graph']}:{c['truth']}:{' '.join(c['flags']) or 'default'}" for c in TRUTH])
-def test_cli_ground_truth_report_matches_reference(tmp_path, idx):
+def test_cli_ground_truth_report_matches_reference(tmp_path, idx, driver):
     """-g FILE [-z]: the comparison report (compare.cpp:274-293) equals the one the compiled
     reference printed for the same graph, truth file and flags; counts exact, scores to
     the last bit."""
+    exe = _driver(driver)
     case = TRUTH[idx]
     g = _build(GOLDEN["graphs"][case["graph"]]["spec"])
     path = str(tmp_path / (case["graph"] + ".bin"))
@@ -91,8 +112,8 @@ def test_cli_ground_truth_report_matches_reference(tmp_path, idx):
         for v, c in enumerate(truth.tolist()):
             f.write(f"{v + 1 if one else v} {c + 1 if one else c}\n")
     env = dict(os.environ, VITE_AMD_SEQUENTIAL_SUMS="1")
-    out = subprocess.run([CLI, "-f", path, "-g", tpath] + (["-z"] if one else []) + case["flags"],
-                         capture_output=True, text=True, env=env, timeout=120)
+    out = subprocess.run([exe, "-f", path, "-g", tpath] + (["-z"] if one else []) + case["flags"],
+                         capture_output=True, text=True, env=env, timeout=120, cwd=str(tmp_path))
     assert out.returncode == 0, out.stderr[-2000:]
     lines = out.stdout.splitlines()
     for key, label in STAT_LINES.items():

@@ -1,8 +1,10 @@
 """GPU parity at larger sizes: a 2^20-vertex RGG (29 M directed entries) swept from a
 mid-run device state, a hub-heavy Barabasi-Albert graph (2^17 vertices, m = 8, rows up to
-a few thousand entries, so the sorted hub path runs) through whole phases and runs, and the
-benchmark's two RGGs (n = 65,536 and n = 2^24 + 1%) through whole runs against the compiled
-reference's main() (tests/golden/reference_large.json).  The graphs are generated on the
+a few thousand entries, so the sorted hub path runs) through whole phases, runs (also with
+coloring and early termination) and a rebuild with a giant community, and the benchmark's
+two RGGs (n = 65,536 and n = 2^24 + 1%) and its skewed-degree BA graph (n = 2^22, m = 8,
+-c 8 -t 2 -a 0.25) through whole runs against the compiled reference's main()
+(tests/golden/reference_large.json).  The graphs are generated on the
 device (the generators' parity is tested at smaller sizes; the benchmark graphs' row offsets
 are checked against the vectors' hashes) and downloaded for the oracle where it is used."""
 import hashlib
@@ -66,6 +68,44 @@ def test_ba_hubs_run_exact(ctx, ba_hubs):
     assert r.level_nv == r_o.level_nv
 
 
+def test_ba_hubs_color_et_run_exact(ctx, ba_hubs):
+    """The skewed-degree configuration's own flags (-c 8 -t 2 -a 0.25: distance-1 coloring in
+    phase 0, early termination type 2) on the hub graph, so the wave-per-row coloring, the
+    long-run event folds and the hub local move run together: every call's Q and iteration
+    count, the level sizes and the color count equal the oracle's."""
+    dg, g = ba_hubs
+    kw = dict(coloring=True, max_colors=8, et_type=2, et_delta=0.25)
+    r_o = po.run(g, **kw)
+    r = dg.run(sequential=True, **kw)
+    assert r.num_colors == r_o.num_colors
+    assert r.call_q == r_o.call_q
+    assert r.call_iters == r_o.call_iters
+    assert r.level_nv == r_o.level_nv
+    assert r.level_ne == r_o.level_ne
+
+
+def test_ba_hubs_giant_community_of_few_hubs_rebuild(ctx, ba_hubs):
+    """A community of at most 64 members holding more than 2^16 entries: the rebuild's
+    overflow path emits it with every wave (k_ovf_emit_big), each lane finding its member by
+    wave shuffles.  The coarse CSR equals the oracle's byte for byte."""
+    dg, g = ba_hubs
+    deg = np.diff(g.offs)
+    order = np.argsort(-deg, kind="stable")
+    k = int(np.searchsorted(np.cumsum(deg[order]), 1 << 16, side="right")) + 1
+    assert 2 <= k <= 64, k
+    cv = np.arange(g.nv, dtype=np.int64)
+    cv[order[:k]] = order[0]  # the k largest hubs in one community, every other vertex alone
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    r = dg.rebuild(cv).download()
+    _, launches, _ = ctx.kernel_stats("rb_ovf_emit_big")
+    ctx.set_profiling(False)
+    assert launches >= 1
+    r_o = po.rebuild(g, cv)
+    assert np.array_equal(r.offs, r_o.offs)
+    assert np.array_equal(r.edges.view(np.uint8), r_o.edges.view(np.uint8))
+
+
 LARGE = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_large.json")))["cases"]
 
 
@@ -115,4 +155,30 @@ def test_rgg_2p24_threshold_cycling_matches_reference(ctx):
     assert r.call_iters == [c["iters"] for c in case["calls"]]
     assert r.level_nv[1:] == [rb["nv"] for rb in case["rebuilds"]][: len(r.level_nv) - 1]
     assert r.level_ne[1:] == [rb["ne"] for rb in case["rebuilds"]][: len(r.level_ne) - 1]
+    dg.free()
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.skipif("ba_2p22_c8t2" not in LARGE, reason="no reference vectors for the 2^22 BA graph")
+def test_ba_2p22_color_et_matches_reference(ctx):
+    """The skewed-degree benchmark graph (Barabasi-Albert n = 2^22, m = 8, 67 M directed
+    entries) with its flags -c 8 -t 2 -a 0.25, against the compiled reference's main() at one
+    thread on the same .bin: with sequential sums every call's Q, iteration count and the
+    coarse graph sizes are equal; with the default tree sums every call's Q is within 1e-6."""
+    case = LARGE["ba_2p22_c8t2"]
+    spec = case["spec"]
+    dg = ctx.gen_ba(spec["seed"], spec["n"], spec["m"])
+    assert (dg.nv, dg.ne) == (case["nv"], case["ne"])
+    assert _offs_sha(dg) == case["offs_sha256"]
+    kw = dict(coloring=True, max_colors=8, et_type=2, et_delta=0.25)
+    r = dg.run(sequential=True, **kw)
+    assert r.call_q == [float.fromhex(c["q_hex"]) for c in case["calls"]]
+    assert r.call_iters == [c["iters"] for c in case["calls"]]
+    assert r.level_nv[1:] == [rb["nv"] for rb in case["rebuilds"]][: len(r.level_nv) - 1]
+    assert r.level_ne[1:] == [rb["ne"] for rb in case["rebuilds"]][: len(r.level_ne) - 1]
+    t = dg.run(**kw)
+    ref = [c["q"] for c in case["calls"]]
+    assert len(t.call_q) == len(ref)
+    assert max(abs(a - b) for a, b in zip(t.call_q, ref)) <= 1e-6
+    assert abs(t.final_q - ref[-1]) <= 1e-6
     dg.free()

@@ -163,3 +163,45 @@ def test_threads_submit_and_wait(ctx):
     with vite_amd.Server(slots=1) as one:
         for r, (rm, _) in zip(reqs, results):
             _same(rm, one.wait(one.submit(*r))[0])
+
+
+# Requests at the serving configuration's sizes: RGG (half with 1% extra edges) and BA
+# (m from 1 to 8), 50/50, n from 2^10 to 2^20.  Big ones first, so the small ones queue
+# behind them and a one-slot server takes them as batched jobs.
+CONFIG_REQS = [("rgg", 5101, 1 << 20, 1.0), ("ba", 5102, 1 << 19, 8), ("rgg", 5103, 1 << 15, 0.0),
+               ("ba", 5104, 1 << 13, 4), ("rgg", 5105, 1 << 10, 1.0), ("ba", 5106, 150000, 2),
+               ("rgg", 5107, 3000, 0.0), ("ba", 5108, 1500, 8)]
+
+
+def _oracle_run(req, kw):
+    from oracle import pyoracle as po
+    kind, seed, n, p = req
+    g = po.gen_rgg(seed, n, p) if kind == "rgg" else po.gen_ba(seed, n, p)
+    return po.run(g, **kw)
+
+
+@pytest.mark.timeout(600)
+def test_serving_matches_oracle_at_config_sizes(ctx):
+    """Every served request's result equals the oracle's run of the same seeded graph: every
+    Louvain call's Q and iteration count, the level sizes and the final Q, with sums in the
+    reference's order.  The same requests are served in a 4-slot mix and on one slot, where
+    the queued small requests run as batched jobs (one vt_batch_run over their union)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import pyoracle as po
+    po._lib()  # load once before the threads
+    kw = dict(sequential=True)
+    with ThreadPoolExecutor(max_workers=8) as pool:  # the oracle's C calls release the GIL
+        expected = list(pool.map(lambda r: _oracle_run(r, {}), CONFIG_REQS))
+    with vite_amd.Server(slots=4) as srv:
+        mixed = [srv.wait(t) for t in [srv.submit(*r, **kw) for r in CONFIG_REQS]]
+    with vite_amd.Server(slots=1) as one:
+        queued = [one.wait(t) for t in [one.submit(*r, **kw) for r in CONFIG_REQS]]
+    for req, want, (rm, im), (rq, iq) in zip(CONFIG_REQS, expected, mixed, queued):
+        for r, info in ((rm, im), (rq, iq)):
+            assert info["status"] == 0, req
+            assert r.call_q == want.call_q, req
+            assert r.call_iters == want.call_iters, req
+            assert r.level_nv == want.level_nv, req
+            assert r.final_q == want.final_q, req
+    assert max(i["batch"] for _, i in queued) > 1  # some small requests ran batched

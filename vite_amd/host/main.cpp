@@ -2,8 +2,9 @@
 // (main.cpp:108-607, getopt string main.cpp:614), running every Louvain phase, rebuild and
 // coloring on one MI355X through the reference-shaped entry points in vite_louvain.hpp.
 //
-// Supported: -f -n -e(0 only) -s -j -i -t -a -c -d -p -h -o -b -r (the last two accepted
-// and ignored at world size 1).  -g/-z (ground-truth comparison) are not provided.
+// Supported: -f -n -e(0 only) -s -j -i -t -a -c -d -p -h -o -g -z -b -r (-b reads through
+// the edge-balanced loader, whose one part at world size 1 is the whole graph; -r, the
+// ranks per node for MPI-IO aggregators, is accepted and has no effect).
 // Workload graphs:  -x "rgg <seed> <n> <extra_pct>"  or  -x "ba <seed> <n> <m>".
 #include <getopt.h>
 
@@ -27,7 +28,7 @@ struct Options {
     std::string input, output_graph, workload;
     bool coloring = false, vertex_order = false, one_phase = false, single_color_iter = false;
     bool generate = false, just_process = false, output_communities = false, threshold_cycling = false;
-    bool early_term = false;
+    bool early_term = false, balanced = false;
     bool compare = false, truth_zero_based = true;  // -g FILE, -z (main.cpp:678-684)
     std::string ground_truth;
     int max_colors = 8, et_type = 0;
@@ -68,7 +69,7 @@ Options parse(int argc, char** argv)
     while ((c = getopt(argc, argv, "f:bc:od:r:t:a:ig:zpn:e:s:jh:x:")) != -1) {
         switch (c) {
             case 'f': o.input = optarg; break;
-            case 'b': break;  // edge-balanced loading: one part at world size 1
+            case 'b': o.balanced = true; break;
             case 'r': break;  // ranks per node: no MPI-IO aggregators here
             case 'c': parse_colors(optarg, o); o.coloring = true; break;
             case 'd': parse_colors(optarg, o); o.vertex_order = true; break;
@@ -129,7 +130,8 @@ DistGraph* make_graph(const Options& o)
         generateInMemGraph(0, 1, dg, o.gen_nv, o.random_edge_pct, o.output_graph);
     } else {
         std::string f = o.input;
-        loadDistGraphMPIIO(0, 1, 1, dg, f);
+        if (o.balanced) loadDistGraphMPIIOBalanced(0, 1, 1, dg, f);
+        else loadDistGraphMPIIO(0, 1, 1, dg, f);
     }
     return dg;
 }
@@ -166,7 +168,7 @@ int main(int argc, char** argv)
     dg->printStats();
     const double td1 = now();
     std::cout << "Time to create distributed graph: " << (td1 - td0) << std::endl;
-    if (!o.output_graph.empty() && !o.generate) writeGraph(*dg, o.output_graph);
+    if (!o.output_graph.empty() && !o.generate) writeGraphBin(*dg, o.output_graph);
     if (o.just_process) {
         std::cout << "Option to just process the graph is turned on, so exiting without invoking community detection."
                   << std::endl;

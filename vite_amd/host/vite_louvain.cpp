@@ -13,6 +13,7 @@
 #include <random>
 #include <sstream>
 #include <algorithm>
+#include <numeric>
 
 #include "../../include/vite_amd.h"
 
@@ -41,8 +42,10 @@ void require_one_rank(int me, int nprocs, const char* fn)
     }
 }
 
-// Device copies of host graphs, keyed by address and checked against a fingerprint of the
-// CSR so a reused address never aliases a different graph.
+// Device copies of host graphs, keyed by address and checked against a hash of the whole
+// CSR (every offset, tail and weight bit), so a reused address or a graph edited in place
+// between calls never reuses a stale device copy.  The hash reads the CSR once at host
+// memory speed (four independent multiply-rotate lanes), about 1 s for a 9 GB graph.
 struct Cached {
     vt_graph* g = nullptr;
     GraphElem nv = -1, ne = -1;
@@ -50,25 +53,38 @@ struct Cached {
 };
 std::map<const DistGraph*, Cached> g_cache;
 
+inline uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+uint64_t hash_words(const void* data, size_t nwords, uint64_t seed)
+{
+    const uint64_t K1 = 0x9E3779B185EBCA87ull, K2 = 0xC2B2AE3D27D4EB4Full;
+    uint64_t lane[4] = {seed + K1 + K2, seed + K2, seed, seed - K1};
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    size_t i = 0;
+    for (; i + 4 <= nwords; i += 4)
+        for (int k = 0; k < 4; k++) {
+            uint64_t w;
+            std::memcpy(&w, p + 8 * (i + k), 8);
+            lane[k] = rotl64(lane[k] + w * K2, 31) * K1;
+        }
+    uint64_t h = rotl64(lane[0], 1) + rotl64(lane[1], 7) + rotl64(lane[2], 12) + rotl64(lane[3], 18);
+    for (; i < nwords; i++) {
+        uint64_t w;
+        std::memcpy(&w, p + 8 * i, 8);
+        h = rotl64(h ^ (rotl64(w * K2, 31) * K1), 27) * K1 + 0x85EBCA77C2B2AE63ull;
+    }
+    h ^= (uint64_t)nwords;
+    h ^= h >> 33;
+    h *= K2;
+    h ^= h >> 29;
+    return h;
+}
+
 uint64_t fingerprint(const Graph& g)
 {
-    uint64_t h = 1469598103934665603ull;
-    auto mix = [&](uint64_t x) {
-        h ^= x;
-        h *= 1099511628211ull;
-    };
     const GraphElem nv = g.getNumVertices(), ne = g.getNumEdges();
-    const GraphElem step_v = nv > 4096 ? nv / 4096 : 1;
-    for (GraphElem i = 0; i <= nv; i += step_v) mix((uint64_t)g.edgeListIndexes[i]);
-    mix((uint64_t)g.edgeListIndexes[nv]);
-    const GraphElem step_e = ne > 4096 ? ne / 4096 : 1;
-    for (GraphElem e = 0; e < ne; e += step_e) {
-        uint64_t w;
-        std::memcpy(&w, &g.edgeList[e].weight, 8);
-        mix((uint64_t)g.edgeList[e].tail);
-        mix(w);
-    }
-    return h;
+    uint64_t h = hash_words(g.edgeListIndexes.data(), (size_t)(nv + 1), (uint64_t)nv);
+    return hash_words(g.edgeList.data(), 2 * (size_t)ne, h ^ (uint64_t)ne);
 }
 
 void forget(const DistGraph* dg)
@@ -147,22 +163,6 @@ vt_ctx* vite_amd_context()
         if (vt_ctx_create(dev, 0, &g_ctx) != VT_OK) fatal("context", "no usable GPU");
     }
     return g_ctx;
-}
-
-void DistGraph::printStats()
-{
-    const Graph& g = getLocalGraph();
-    long ne = (long)g.getNumEdges(), nv = (long)getTotalNumVertices();
-    std::cout << std::endl
-              << "-------------------------------------------------------" << std::endl
-              << "Graph edge distribution characteristics" << std::endl
-              << "-------------------------------------------------------" << std::endl
-              << "Number of vertices: " << nv << std::endl
-              << "Number of edges: " << ne << std::endl
-              << "Maximum number of edges: " << ne << std::endl
-              << "Average number of edges: " << (double)ne << std::endl
-              << "Minimum number of edges: " << ne << std::endl
-              << "-------------------------------------------------------" << std::endl;
 }
 
 // ---- the nine Louvain overloads -----------------------------------------------------------
@@ -311,35 +311,135 @@ ColorElem distColoringMultiHashMinMax(const int me, const int nprocs, const Dist
 
 // ---- graph input / generation ---------------------------------------------------------------
 
-void loadDistGraphMPIIO(int me, int nprocs, int, DistGraph*& dg, std::string& fileName)
+// The .bin layout (distgraph.cpp:119-253): {nv, ne} as two GraphElem, nv+1 offsets, then
+// 16-byte edge records from offset 16 + 8(nv+1).  At world size 1 the one rank reads
+// every row; its edge count is offsets[nv] - offsets[0] and the offsets are rebased to 0,
+// as the reference does after its read (distgraph.cpp:208-247).
+static void read_bin(const char* fn, std::string& fileName, DistGraph*& dg, bool announce)
 {
-    require_one_rank(me, nprocs, "loadDistGraphMPIIO");
-    std::ifstream f(fileName, std::ios::binary);
-    if (!f) fatal("loadDistGraphMPIIO", (" Error opening file! " + fileName).c_str());
-    GraphElem hdr[2];
-    f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
-    if (!f || hdr[0] < 0 || hdr[1] < 0) fatal("loadDistGraphMPIIO", "bad header");
-    std::cout << "Reading file of " << hdr[0] << " vertices and " << hdr[1] << " edges." << std::endl;
-    dg = new DistGraph(hdr[0], hdr[1]);
-    PartRanges parts{0, hdr[0]};
-    dg->createLocalGraph(hdr[0], hdr[1], &parts);
+    std::FILE* fp = std::fopen(fileName.c_str(), "rb");
+    if (!fp) fatal(fn, (" Error opening file! " + fileName).c_str());
+    GraphElem hdr[2] = {-1, -1};
+    if (std::fread(hdr, sizeof(GraphElem), 2, fp) != 2 || hdr[0] < 0 || hdr[1] < 0) {
+        std::fclose(fp);
+        fatal(fn, ("bad header in " + fileName).c_str());
+    }
+    const GraphElem nv = hdr[0];
+    if (announce) std::cout << "Reading file of " << nv << " vertices and " << hdr[1] << " edges." << std::endl;
+    dg = new DistGraph(nv, hdr[1]);
+    PartRanges parts{0, nv};
+    dg->createLocalGraph(nv, 0, &parts);
     Graph& g = dg->getLocalGraph();
-    f.read(reinterpret_cast<char*>(g.edgeListIndexes.data()), sizeof(GraphElem) * (hdr[0] + 1));
-    f.read(reinterpret_cast<char*>(g.edgeList.data()), sizeof(Edge) * hdr[1]);
-    if (!f) fatal("loadDistGraphMPIIO", "short read");
-    const GraphElem base = g.edgeListIndexes[0];  // rebase (distgraph.cpp:244-247)
+    if (std::fread(g.edgeListIndexes.data(), sizeof(GraphElem), (size_t)nv + 1, fp) != (size_t)nv + 1) {
+        std::fclose(fp);
+        fatal(fn, "short read of the offsets");
+    }
+    const GraphElem base = g.edgeListIndexes[0];
+    const GraphElem lne = g.edgeListIndexes[nv] - base;
+    if (base < 0 || lne < 0) {
+        std::fclose(fp);
+        fatal(fn, "offsets out of order");
+    }
+    g.setNumEdges(lne);
+    const long off = (long)(2 + nv + 1) * (long)sizeof(GraphElem) + (long)base * (long)sizeof(Edge);
+    if (std::fseek(fp, off, SEEK_SET) != 0 ||
+        std::fread(g.edgeList.data(), sizeof(Edge), (size_t)lne, fp) != (size_t)lne) {
+        std::fclose(fp);
+        fatal(fn, "short read of the edge records");
+    }
+    std::fclose(fp);
     for (auto& o : g.edgeListIndexes) o -= base;
 }
 
-void writeGraph(const DistGraph& dg, const std::string& fileName)
+void loadDistGraphMPIIO(int me, int nprocs, int, DistGraph*& dg, std::string& fileName)
+{
+    require_one_rank(me, nprocs, "loadDistGraphMPIIO");
+    read_bin("loadDistGraphMPIIO", fileName, dg, true);
+}
+
+void balanceEdges(int nprocs, std::string& fileName, std::vector<GraphElem>& mbins)
+{
+    // distgraph.cpp:72-116: scan the offsets once, filling part p until it holds
+    // ne / nprocs edges (the last part takes the rest); mbins gets the parts' first rows.
+    std::FILE* fp = std::fopen(fileName.c_str(), "rb");
+    if (!fp) fatal("balanceEdges", (" Error opening file! " + fileName).c_str());
+    GraphElem hdr[2] = {0, 0};
+    if (std::fread(hdr, sizeof(GraphElem), 2, fp) != 2) {
+        std::fclose(fp);
+        fatal("balanceEdges", "bad header");
+    }
+    std::vector<GraphElem> nbins(nprocs, 0);
+    if ((int)mbins.size() < nprocs + 1) mbins.assign(nprocs + 1, 0);
+    const GraphElem cap = hdr[1] / nprocs;
+    GraphElem prev = 0, cur = 0;
+    int p = 0;
+    for (GraphElem m = 0; m < hdr[0]; m++) {
+        if (std::fread(&cur, sizeof(GraphElem), 1, fp) != 1) {
+            std::fclose(fp);
+            fatal("balanceEdges", "short read of the offsets");
+        }
+        if (nbins[p] < cap || p == nprocs - 1) nbins[p] += cur - prev;
+        if (nbins[p] >= cap && p < nprocs - 1) p++;
+        mbins[p + 1]++;
+        prev = cur;
+    }
+    std::fclose(fp);
+    for (int k = 1; k < nprocs + 1; k++) mbins[k] += mbins[k - 1];
+}
+
+void loadDistGraphMPIIOBalanced(int me, int nprocs, int, DistGraph*& dg, std::string& fileName)
+{
+    // distgraph.cpp:256-387: the edge-balanced split has one part at world size 1, so the
+    // rows read are those of the plain read.
+    require_one_rank(me, nprocs, "loadDistGraphMPIIOBalanced");
+    std::vector<GraphElem> mbins(nprocs + 1, 0);
+    balanceEdges(nprocs, fileName, mbins);
+    std::cout << "Trying to achieve equal edge distribution across processes." << std::endl;
+    read_bin("loadDistGraphMPIIOBalanced", fileName, dg, false);
+    if (mbins[1] != dg->getLocalGraph().getNumVertices()) fatal("loadDistGraphMPIIOBalanced", "part bounds differ");
+}
+
+void writeGraph(int me, int nprocs, DistGraph*& dg, std::vector<GraphElem>& edgeCount, std::string& fileName)
+{
+    // distgraph.cpp:986-1064 at world size 1: edgeCount holds each row's degree at [v+1]
+    // (and 0 at [0]); its prefix sums are the offsets written.
+    require_one_rank(me, nprocs, "writeGraph");
+    const Graph& g = dg->getLocalGraph();
+    const GraphElem nv = dg->getTotalNumVertices();
+    if ((GraphElem)edgeCount.size() < nv + 1) fatal("writeGraph", "edgeCount needs nv + 1 entries");
+    std::vector<GraphElem> offs(nv + 1);
+    std::partial_sum(edgeCount.begin(), edgeCount.begin() + nv + 1, offs.begin());
+    GraphElem hdr[2] = {nv, (GraphElem)g.edgeList.size()};
+    std::FILE* fp = std::fopen(fileName.c_str(), "wb");
+    if (!fp) fatal("writeGraph", (" Error opening output binary file for storing graph data! " + fileName).c_str());
+    const bool ok = std::fwrite(hdr, sizeof(GraphElem), 2, fp) == 2 &&
+                    std::fwrite(offs.data(), sizeof(GraphElem), (size_t)nv + 1, fp) == (size_t)nv + 1 &&
+                    std::fwrite(g.edgeList.data(), sizeof(Edge), (size_t)hdr[1], fp) == (size_t)hdr[1];
+    if (std::fclose(fp) != 0 || !ok) fatal("writeGraph", ("short write to " + fileName).c_str());
+}
+
+void writeGraphBin(const DistGraph& dg, const std::string& fileName)
 {
     const Graph& g = dg.getLocalGraph();
-    std::ofstream f(fileName, std::ios::binary);
-    if (!f) fatal("writeGraph", (" Error opening output binary file for storing graph data! " + fileName).c_str());
-    GraphElem hdr[2] = {g.getNumVertices(), g.getNumEdges()};
-    f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
-    f.write(reinterpret_cast<const char*>(g.edgeListIndexes.data()), sizeof(GraphElem) * (hdr[0] + 1));
-    f.write(reinterpret_cast<const char*>(g.edgeList.data()), sizeof(Edge) * hdr[1]);
+    const GraphElem nv = g.getNumVertices();
+    std::vector<GraphElem> edgeCount(nv + 1, 0);
+    for (GraphElem v = 0; v < nv; v++) edgeCount[v + 1] = g.edgeListIndexes[v + 1] - g.edgeListIndexes[v];
+    DistGraph* p = const_cast<DistGraph*>(&dg);
+    std::string fn = fileName;
+    writeGraph(0, 1, p, edgeCount, fn);
+}
+
+// gatherAllComm at world size 1 (louvain.cpp:3439-3480): the root's own vector.
+void gatherAllComm(int root, int me, int nprocs, std::vector<GraphElem>& commAll, const CommunityVector& localComm)
+{
+    require_one_rank(me, nprocs, "gatherAllComm");
+    if (root != 0) fatal("gatherAllComm", "root must be rank 0");
+    commAll.assign(localComm.begin(), localComm.end());
+}
+
+void vite_amd_release(const DistGraph* dg)
+{
+    if (g_ctx) forget(dg);
 }
 
 // ---- ground truth ---------------------------------------------------------------------------
@@ -426,7 +526,7 @@ void generateInMemGraph(int rank, int nprocs, DistGraph*& dg, GraphElem nv, Grap
     check(vt_gen_vite_rgg(g_ctx, nv, seeds[0], &g), "generateInMemGraph");
     dg = host_from_device(g);
     if (!fileOut.empty()) {
-        writeGraph(*dg, fileOut);
+        writeGraphBin(*dg, fileOut);
         std::cout << "Written binary file: " << fileOut << std::endl;
     }
 }

@@ -6,8 +6,9 @@
 // Errors follow the reference's convention: a message on stderr, then the process exits
 // (the reference calls MPI_Abort(-99), main.cpp:714-768).
 //
-// A DistGraph handed to these functions is treated as immutable afterwards: its device copy
-// is cached by address and reused by later calls (the Louvain phase, then the rebuild).
+// The device copy of a DistGraph is cached by address and reused by later calls (the
+// Louvain phase, then the rebuild) while a hash of its whole CSR is unchanged; a graph
+// edited in place between calls is uploaded again.
 #pragma once
 
 #include <string>
@@ -86,10 +87,17 @@ ColorElem distColoringMultiHashMinMax(const int me, const int nprocs, const Dist
                                       const ColorElem nHash, const int target_percent, const bool singleIteration);
 
 // ---- distgraph.hpp:109-119 ------------------------------------------------------------
+void balanceEdges(int nprocs, std::string& fileName, std::vector<GraphElem>& mbins);
 void loadDistGraphMPIIO(int me, int nprocs, int ranks_per_node, DistGraph*& dg, std::string& fileName);
+void loadDistGraphMPIIOBalanced(int me, int nprocs, int ranks_per_node, DistGraph*& dg, std::string& fileName);
 void generateInMemGraph(int rank, int nprocs, DistGraph*& dg, GraphElem nv, GraphWeight randomEdgePercent,
                         std::string fileOut);
-void writeGraph(const DistGraph& dg, const std::string& fileName);
+void writeGraph(int me, int nprocs, DistGraph*& dg, std::vector<GraphElem>& edgeCount, std::string& fileName);
+// The same .bin file from the graph's own offsets (no per-row counts needed).
+void writeGraphBin(const DistGraph& dg, const std::string& fileName);
+
+// ---- louvain.hpp:206-208 -----------------------------------------------------------------
+void gatherAllComm(int root, int me, int nprocs, std::vector<GraphElem>& commAll, const CommunityVector& localComm);
 
 // ---- ground truth: louvain.hpp:201-203, compare.hpp:56-57 -----------------------------
 // LFR-style "vertex community" lines; community ids one-based unless zero-based is set.
@@ -113,3 +121,6 @@ void destroyEdgeMPIType();
 // Library context used by the functions above (device 0 unless VITE_AMD_DEVICE is set).
 struct vt_ctx;
 vt_ctx* vite_amd_context();
+// Frees the device copy of dg, if one is cached (call before deleting a graph whose device
+// copy should not outlive it; a later call on the same address uploads it again).
+void vite_amd_release(const DistGraph* dg);
