@@ -264,9 +264,10 @@ def hbm_table(kt_dir, fetch_dir, write_dir, top=30):
     return "\n".join(lines) + "\n"
 
 
-def reference_run(args):
-    """The reference main() run on the benchmark graph, when the committed vectors cover this
-    exact input (seed, n, extra-edge percentage): per-call Q, iterations and wall time."""
+def reference_run(kind, seed, n, param, flags):
+    """The reference main() run on a benchmark graph, when the committed vectors cover this
+    exact input (generator, seed, n, extra-edge percentage for RGG or m for BA, flags):
+    per-call Q, iterations and wall time."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "reference_large.json")
     try:
         cases = json.load(open(path))["cases"]
@@ -274,10 +275,20 @@ def reference_run(args):
         return None
     for case in cases.values():
         sp = case["spec"]
-        if (sp["seed"], sp["n"], sp["extra_pct"], sp["flags"]) == (args.seed, args.n, args.extra, ["-i"]):
+        ckind = sp.get("kind", "rgg")
+        cparam = sp.get("m") if ckind == "ba" else sp.get("extra_pct")
+        if (ckind, sp["seed"], sp["n"], cparam, sp["flags"]) == (kind, seed, n, param, list(flags)):
             return {"q": [c["q"] for c in case["calls"]], "iters": [c["iters"] for c in case["calls"]],
                     "seconds": case.get("reference_seconds"), "threads": sp["threads"]}
     return None
+
+
+def compare_reference(r, ref, source):
+    """A device run's per-call Q and iterations against the reference's on the same graph."""
+    dq = max(abs(a - b) for a, b in zip(r.call_q, ref["q"])) if len(ref["q"]) == len(r.call_q) else None
+    return {"final_q": ref["q"][-1], "abs_diff_final_q": abs(r.final_q - ref["q"][-1]), "max_abs_diff_call_q": dq,
+            "same_iterations": ref["iters"] == r.call_iters, "build_container_seconds": ref["seconds"],
+            "source": source}
 
 
 def batched_form(ctx, args):
@@ -373,11 +384,17 @@ def skewed_form(ctx, args):
         f"-c 8 -t 2 -a 0.25 run {run_s:.2f}s Q={r.final_q:.9f}")
     # the same algorithmic bytes as the headline (24E + 56V) over the whole sweep's wall time
     sweep_bytes = 24.0 * float(deg.sum()) + 56.0 * float(len(deg))
-    return {"n": int(len(deg)), "directed_entries": int(deg.sum()), "max_degree": int(deg.max()),
-            "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall,
-            "sweep_hbm_frac": sweep_bytes / wall / 1e9 / HBM_PEAK_GBS, "kernel_avg_ms": kernels,
-            "run_seconds": run_s, "final_q": r.final_q, "levels": len(r.level_q), "colors": r.num_colors,
-            "flags": "-c 8 -t 2 -a 0.25", "tuning": tuning}
+    out = {"n": int(len(deg)), "directed_entries": int(deg.sum()), "max_degree": int(deg.max()),
+           "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall,
+           "sweep_hbm_frac": sweep_bytes / wall / 1e9 / HBM_PEAK_GBS, "kernel_avg_ms": kernels,
+           "run_seconds": run_s, "final_q": r.final_q, "levels": len(r.level_q), "colors": r.num_colors,
+           "flags": "-c 8 -t 2 -a 0.25", "tuning": tuning}
+    ref = reference_run("ba", args.seed + 3, args.skewed_n, 8, ["-c", "8", "-t", "2", "-a", "0.25"])
+    if ref:
+        out["reference"] = compare_reference(
+            r, ref, "the reference's main() -c 8 -t 2 -a 0.25 on the same graph (written to .bin), 1 MPI rank, "
+                    f"{ref['threads']} OpenMP thread, run once in the build container")
+    return out
 
 
 def serving_form(args):
@@ -598,15 +615,12 @@ def main():
         out["time_to_final_q"] = {"seconds": ttq, "final_q": r.final_q, "levels": len(r.level_q),
                                   "calls": len(r.call_q), "total_iters": r.total_iters,
                                   "clustering_seconds": r.clustering_seconds, "flags": "-i"}
-        ref = reference_run(args)
+        ref = reference_run("rgg", args.seed, args.n, args.extra, ["-i"])
         if ref:
-            dq = max(abs(a - b) for a, b in zip(r.call_q, ref["q"])) if len(ref["q"]) == len(r.call_q) else None
-            out["time_to_final_q"]["reference"] = {
-                "final_q": ref["q"][-1], "abs_diff_final_q": abs(r.final_q - ref["q"][-1]), "max_abs_diff_call_q": dq,
-                "same_iterations": ref["iters"] == r.call_iters, "build_container_seconds": ref["seconds"],
-                "source": "the reference's main() -i on the same graph, 1 MPI rank, run once in the build "
-                          f"container ({ref['threads']} OpenMP threads), NOT on this host; the same-host "
-                          "comparison is time_to_final_q_config1"}
+            out["time_to_final_q"]["reference"] = compare_reference(
+                r, ref, "the reference's main() -i on the same graph, 1 MPI rank, run once in the build "
+                        f"container ({ref['threads']} OpenMP threads), NOT on this host; the same-host "
+                        "comparison is time_to_final_q_config1")
         log(f"time to final Q: {ttq:.2f}s q={r.final_q:.9f} levels={len(r.level_q)} iters={r.total_iters}")
 
     if args.skewed_n > 0:
