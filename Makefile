@@ -8,7 +8,7 @@ HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -ffp-contract=off \
             -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-result
 CXXFLAGS := -O2 -std=c++17 -fPIC -ffp-contract=off -Wall
 LIB      := vite_amd/libvite_amd.so
-SRCS     := vite_amd/csrc/vt_core.hip vite_amd/csrc/vt_louvain.hip vite_amd/csrc/vt_rebuild.hip vite_amd/csrc/vt_color.hip vite_amd/csrc/vt_gen.hip vite_amd/csrc/vt_driver.hip vite_amd/csrc/vt_batch.hip vite_amd/csrc/vt_server.hip vite_amd/csrc/vt_compare.hip
+SRCS     := vite_amd/csrc/vt_core.hip vite_amd/csrc/vt_louvain.hip vite_amd/csrc/vt_rebuild.hip vite_amd/csrc/vt_color.hip vite_amd/csrc/vt_gen.hip vite_amd/csrc/vt_driver.hip vite_amd/csrc/vt_batch.hip vite_amd/csrc/vt_server.hip vite_amd/csrc/vt_compare.hip vite_amd/csrc/vt_dsort.hip
 OBJS     := $(SRCS:.hip=.o)
 HDRS     := include/vite_amd.h vite_amd/csrc/vt_internal.hpp
 

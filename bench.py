@@ -374,7 +374,14 @@ def skewed_form(ctx, args):
         if cnt:
             kernels[k] = avg
     ctx.set_profiling(False)
+    # the sweep's reads at random places: one community read per entry (curr[tail]) and one
+    # community-degree read per distinct neighbour community of each vertex (K of them)
+    curr = st.state()["curr"]
     st.end(want_cvect=False)
+    tails = dg.download().edges["tail"]
+    heads = np.repeat(np.arange(len(deg), dtype=np.int64), deg)
+    K = int(len(np.unique(heads * len(deg) + curr[tails])))
+    del heads, tails
     t0 = time.perf_counter()
     r = dg.run(coloring=True, max_colors=8, et_type=2, et_delta=0.25)
     run_s = time.perf_counter() - t0
@@ -387,6 +394,8 @@ def skewed_form(ctx, args):
     out = {"n": int(len(deg)), "directed_entries": int(deg.sum()), "max_degree": int(deg.max()),
            "sweep_ms": 1e3 * wall, "sweep_edges_per_s": float(deg.sum()) / wall,
            "sweep_hbm_frac": sweep_bytes / wall / 1e9 / HBM_PEAK_GBS, "kernel_avg_ms": kernels,
+           "candidate_communities": K, "random_reads": int(deg.sum()) + K,
+           "random_reads_per_s": (float(deg.sum()) + K) / wall,
            "run_seconds": run_s, "final_q": r.final_q, "levels": len(r.level_q), "colors": r.num_colors,
            "flags": "-c 8 -t 2 -a 0.25", "tuning": tuning}
     ref = reference_run("ba", args.seed + 3, args.skewed_n, 8, ["-c", "8", "-t", "2", "-a", "0.25"])
@@ -489,6 +498,51 @@ def measured_traffic(kernel, workload_tag, symbol):
     return 1024.0 * (vals["FETCH_SIZE"] * FETCH_SCALE + vals["WRITE_SIZE"] * WRITE_SCALE), os.path.relpath(path, ROOT)
 
 
+def gather_probe(args):
+    """The random-read floor of the skewed run: one 4-byte read of a per-vertex array at
+    every CSR entry's tail (curr[tail], as the local move reads each neighbour's community),
+    done by a plain device gather (torch.index_select) with nothing else in the kernel.  The
+    same reads with the vertices renumbered by degree (hubs first) and at random show how
+    much vertex order matters.  Mean of 10 launches each, timed with device events."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    # the graph comes from the library's device generator in a child process, so that this
+    # process touches the GPU through torch only
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "ba.npz")
+        code = ("import sys, numpy as np; sys.path.insert(0, %r); import vite_amd; "
+                "ctx = vite_amd.Context(0); g = ctx.gen_ba(%d, %d, 8).download(); "
+                "np.savez(%r, offs=g.offs, tails=g.edges['tail'])") % (
+                    os.path.dirname(os.path.abspath(__file__)), args.seed + 3, args.skewed_n, path)
+        subprocess.run([sys.executable, "-c", code], check=True)
+        z = np.load(path)
+        offs, tails = z["offs"], z["tails"].astype(np.int64)
+    n, ne = len(offs) - 1, len(tails)
+    deg = np.diff(offs)
+    rank = np.empty(n, np.int64)
+    rank[np.argsort(-deg, kind="stable")] = np.arange(n)
+    perm = np.random.default_rng(args.seed).permutation(n)
+    curr = torch.arange(n, dtype=torch.int32, device="cuda")
+    out = {"probe": "gather", "graph": f"BA n={n} m=8 seed {args.seed + 3}", "reads": ne}
+    for label, t in (("as_generated", tails), ("degree_ranked", rank[tails]), ("random_order", perm[tails])):
+        idx = torch.from_numpy(np.ascontiguousarray(t)).to("cuda")
+        torch.index_select(curr, 0, idx)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            torch.index_select(curr, 0, idx)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        out[label] = {"ms": ms, "reads_per_s": ne / (ms * 1e-3)}
+        log(f"gather probe {label}: {ms:.3f} ms, {ne / ms / 1e6:.1f} G reads/s")
+    return out
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--summarize-profile":
         sys.stdout.write(summarize_profile(sys.argv[2]))
@@ -520,7 +574,14 @@ def main():
                     help="offered load as a fraction of the sequential solver's measured capacity")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration)")
+    ap.add_argument("--probe-gather", action="store_true",
+                    help="only time plain random 4-byte gathers curr[tail] over the skewed run's BA graph "
+                         "(the floor of its local move's neighbour-community reads)")
     args = ap.parse_args()
+
+    if args.probe_gather:
+        print(json.dumps(gather_probe(args)), flush=True)
+        return
 
     if args.probe_copy:
         import torch

@@ -162,6 +162,7 @@ int vt_sweep(vt_ctx* ctx, const vt_graph* g, const int64_t* curr, const int64_t*
 int vt_modularity(vt_ctx* ctx, int64_t nv, const double* cw, const double* cdeg, double constant,
                   int32_t flags, double* q_out);
 
+
 /* Vertex degrees (row left folds) and 1/2m. */
 int vt_init_degrees(vt_ctx* ctx, const vt_graph* g, int32_t flags, double* vdeg_out,
                     double* constant_out);
@@ -205,6 +206,17 @@ int vt_gen_ba(vt_ctx* ctx, uint64_t seed, int64_t n, int64_t m, vt_graph** out);
  * Y = next n, radius (sqrt(ln n/(3.14159 n)) + sqrt(2.0736/n))/2, rows sorted by tail.
  * The reference draws x0 as the first word of std::seed_seq{1}. */
 int vt_gen_vite_rgg(vt_ctx* ctx, int64_t n, uint32_t x0, vt_graph** out);
+
+/* ---- device primitives of the batched loop ---------------------------------------- */
+/* No reference counterpart: the scan and the stable radix sort the batched loop runs inside
+ * its captured HIP graph, where the element count is only known on the device.  Exposed so
+ * they can be tested on their own.  Host arrays in and out; the count n is handed to the
+ * kernels through device memory, and the grids are sized for `capacity` (>= n).
+ * vt_dev_sort_pairs: stable sort of n (key, value) pairs by key bits [0, bits), 1 <= bits <= 32.
+ * vt_dev_scan: exclusive prefix sums of n int32 values; total_out (may be NULL) gets the sum. */
+int vt_dev_sort_pairs(vt_ctx* ctx, int64_t n, int64_t capacity, int32_t bits, const uint32_t* keys,
+                      const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out);
+int vt_dev_scan(vt_ctx* ctx, int64_t n, int64_t capacity, const int32_t* in, int32_t* out, int64_t* total_out);
 
 /* ---- whole run (main.cpp:242-513) -------------------------------------------------- */
 typedef struct {

@@ -53,6 +53,10 @@ def test_batch_equals_one_by_one(ctx, mixed, kw, use_graph, sequential):
     res, st = b.run(use_graph=use_graph, sequential=sequential, **kw)
     b.free()
     assert st["phases"] >= 1 and st["rounds"] >= st["phases"]
+    # phases run in lockstep, call k of every graph in batch phase k, and a phase lasts as
+    # many rounds as its longest-running graph iterates (no round is counted twice or lost)
+    calls = max(len(r.call_iters) for r in res)
+    assert st["rounds"] == sum(max(r.call_iters[k] for r in res if len(r.call_iters) > k) for k in range(calls))
     for g, r in zip(dev, res):
         _same(r, g.run(sequential=sequential, **kw))
 

@@ -121,5 +121,8 @@ def test_cli_ground_truth_report_matches_reference(tmp_path, idx, driver):
         want = case["stats"][key]
         if isinstance(want, int):
             assert int(val) == want, key
+        elif driver == "dropin":
+            # the reference's driver leaves std::cout at its default six significant digits
+            assert val == format(float.fromhex(want), ".6g"), key
         else:
             assert float(val).hex() == want, key

@@ -64,8 +64,13 @@ def test_admission_order_and_budget(ctx):
     with vite_amd.Server(hbm_budget=budget, slots=4) as srv:
         tickets = [srv.submit(*r, threshold_cycling=True) for r in reqs]
         infos = [srv.wait(t)[1] for t in tickets]
-    starts = [i["start_s"] for i in infos]
-    assert starts == sorted(starts)  # admitted in arrival order
+    # each admission takes the oldest queued request (a batched job's head, its
+    # lowest-numbered member; its other members may have arrived later)
+    groups = {}
+    for k, i in enumerate(infos):
+        groups.setdefault((i["slot"], i["start_s"]), []).append(k)
+    heads = [min(m) for _, m in sorted(groups.items(), key=lambda kv: kv[0][1])]
+    assert heads == sorted(heads)
     for i in infos:
         assert i["status"] == 0
         assert i["peak_bytes"] <= i["reserved_bytes"]
@@ -102,6 +107,31 @@ def test_queued_small_requests_batched(ctx, kw):
         g.free()
     _same(r1[0], out[0][0])
     assert r2[1]["start_s"] != r1[1]["start_s"] and r2[1]["start_s"] != r3[1]["start_s"]
+
+
+def test_small_requests_batched_past_a_large_one(ctx):
+    """A batched job takes the matching small requests wherever they wait in the queue: with a
+    large request queued between small ones, the small ones behind it join the job of the
+    small one ahead of it, and the large one runs after that job, as the oldest request left."""
+    kw = dict(threshold_cycling=True)
+    big = [("rgg", 31, 40000, 1.0), ("rgg", 33, 30000, 0.0)]
+    small = [("rgg", 32, 900, 1.0), ("ba", 34, 700, 4), ("rgg", 35, 1500, 0.0)]
+    with vite_amd.Server(slots=1, max_batch=8, batch_max_n=3000) as srv:
+        t_big0 = srv.submit(*big[0], **kw)  # holds the one slot while the others queue
+        t_s0 = srv.submit(*small[0], **kw)
+        t_big1 = srv.submit(*big[1], **kw)
+        t_s1 = srv.submit(*small[1], **kw)
+        t_s2 = srv.submit(*small[2], **kw)
+        out = {t: srv.wait(t) for t in (t_big0, t_s0, t_big1, t_s1, t_s2)}
+    start = {t: out[t][1]["start_s"] for t in out}
+    assert out[t_s0][1]["batch"] == 3
+    assert start[t_s0] == start[t_s1] == start[t_s2]
+    assert start[t_big0] < start[t_s0] < start[t_big1]
+    for t, req in zip((t_big0, t_s0, t_big1, t_s1, t_s2), (big[0], small[0], big[1], small[1], small[2])):
+        kind, seed, n, p = req
+        g = ctx.gen_rgg(seed, n, p) if kind == "rgg" else ctx.gen_ba(seed, n, p)
+        _same(out[t][0], g.run(**kw))
+        g.free()
 
 
 def test_batching_settings_rejected(ctx):

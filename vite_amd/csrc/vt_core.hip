@@ -247,14 +247,24 @@ __global__ void __launch_bounds__(kSumBlock) k_sum2_final(const double2* __restr
     if (threadIdx.x == 0) { out[0] = sa[0]; out[1] = sb[0]; }
 }
 
-void tree_sum2(vt_ctx* ctx, const double* a, const double* b, int64_t n, double* d_out2)
+// The blocking of an n-element tree sum: at most kSumGrid blocks, four strides of 256
+// elements per block at least.  A function of n alone.
+static void sum2_plan(int64_t n, int64_t* grid_out, int64_t* chunk_out)
 {
     int64_t grid = (n + 4 * kSumBlock - 1) / (4 * kSumBlock);
     if (grid < 1) grid = 1;
     if (grid > kSumGrid) grid = kSumGrid;
     const int64_t chunk = (n + grid - 1) / grid;
+    *grid_out = grid;
+    *chunk_out = chunk > 0 ? chunk : 1;
+}
+
+void tree_sum2(vt_ctx* ctx, const double* a, const double* b, int64_t n, double* d_out2)
+{
+    int64_t grid = 0, chunk = 0;
+    sum2_plan(n, &grid, &chunk);
     double2* part = (double2*)scratch(ctx, "sum2_part", sizeof(double2) * kSumGrid);
-    VT_LAUNCH(ctx, "sum2_partial", k_sum2_partial, (unsigned)grid, kSumBlock, 0, a, b, n, chunk > 0 ? chunk : 1, part);
+    VT_LAUNCH(ctx, "sum2_partial", k_sum2_partial, (unsigned)grid, kSumBlock, 0, a, b, n, chunk, part);
     VT_LAUNCH(ctx, "sum2_final", k_sum2_final, 1, kSumBlock, 0, (const double2*)part, (int)grid, d_out2);
 }
 

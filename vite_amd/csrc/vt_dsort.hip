@@ -1,0 +1,304 @@
+// Scan and stable radix sort whose element count lives in device memory.
+//
+// The batched phase loop replays each Louvain round from one captured HIP graph.  The round's
+// event count (two events per vertex that moves) is only known on the device, so the sort of
+// the events cannot take its size from the host.  These kernels read the count from device
+// memory, size their grids for a host-known capacity and let the blocks past the count exit
+// at once.  They record no memset, allocate nothing once their scratch exists, and make no
+// host read, so a sequence of them can be captured and replayed.
+//
+// Sort: least-significant-digit radix sort, 8-bit digits, tiles of 2048 pairs.  Per pass a
+// histogram kernel counts each tile's digits (digit-major table: digit d of tile b at
+// d * ntiles + b), the table is scanned, and a scatter kernel places each pair at its digit's
+// offset for its tile plus its rank among the tile's pairs of that digit in input order.
+// Ranks come from wave ballots over the digit bits, so equal keys keep their input order
+// (stable), which is what the per-community event folds need.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "vt_internal.hpp"
+
+namespace vt {
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int64_t kScanChunk = (int64_t)kScanBlock * kScanItems;  // 2048 values per block
+constexpr int kPartBlock = 1024;
+
+constexpr int kSortBlock = 256;
+constexpr int kSortRounds = 8;
+constexpr int64_t kTile = (int64_t)kSortBlock * kSortRounds;  // 2048 pairs per tile
+constexpr int kWaves = kSortBlock / 64;
+
+__device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Inclusive prefix sum over the 256 threads of a block; returns the thread's inclusive sum
+// and leaves the block total in *total.
+__device__ __forceinline__ int64_t block_inclusive_scan(int64_t x, int64_t* s_wave, int64_t* total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[w] = x;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+    for (int k = 0; k < kScanBlock / 64; k++) {
+        if (k < w) before += s_wave[k];
+        all += s_wave[k];
+    }
+    __syncthreads();
+    *total = all;
+    return x + before;
+}
+
+// Block partial sums of in[0, n).
+__global__ void __launch_bounds__(kScanBlock) k_dscan_reduce(const int32_t* __restrict__ in, const int64_t* __restrict__ d_n,
+                                                           int64_t* __restrict__ part)
+{
+    __shared__ int64_t s_wave[kScanBlock / 64];
+    const int64_t n = *d_n;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    if (base >= n) return;
+    int64_t x = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        const int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
+        if (i < n) x += in[i];
+    }
+    int64_t total = 0;
+    block_inclusive_scan(x, s_wave, &total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block partials in place (one block), and the grand total.
+__global__ void __launch_bounds__(kPartBlock) k_dscan_parts(const int64_t* __restrict__ d_n, int64_t* __restrict__ part,
+                                                          int64_t* __restrict__ d_total)
+{
+    __shared__ int64_t s_wave[kPartBlock / 64];
+    const int64_t nb = ceil_div(*d_n, kScanChunk);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += kPartBlock) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t v = i < nb ? part[i] : 0;
+        int64_t x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_wave[w] = x;
+        __syncthreads();
+        int64_t before = 0, all = 0;
+        for (int k = 0; k < kPartBlock / 64; k++) {
+            if (k < w) before += s_wave[k];
+            all += s_wave[k];
+        }
+        if (i < nb) part[i] = carry + before + x - v;
+        carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+// Exclusive scan of each block's values, offset by the block's scanned partial.
+__global__ void __launch_bounds__(kScanBlock) k_dscan_down(const int32_t* __restrict__ in, const int64_t* __restrict__ d_n,
+                                                         const int64_t* __restrict__ part, int32_t* __restrict__ out)
+{
+    __shared__ int64_t s_wave[kScanBlock / 64];
+    const int64_t n = *d_n;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    if (base >= n) return;
+    // thread t scans the contiguous run [t*8, t*8+8) of the chunk
+    const int64_t lo = base + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int64_t x = 0;
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = (lo + k < n) ? in[lo + k] : 0;
+        x += v[k];
+    }
+    int64_t total = 0;
+    int64_t run = block_inclusive_scan(x, s_wave, &total) - x + part[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++) {
+        if (lo + k < n) out[lo + k] = (int32_t)run;
+        run += v[k];
+    }
+}
+
+// Sizes of one sort: pairs, tiles and histogram entries (256 per tile).
+__global__ void k_dsort_sizes(const int64_t* __restrict__ d_count, int64_t mult, int64_t* __restrict__ sizes)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t n = *d_count * mult;
+        const int64_t nt = ceil_div(n, kTile);
+        sizes[0] = n;
+        sizes[1] = nt;
+        sizes[2] = 256 * nt;
+    }
+}
+
+// The pass's digit: key bits [shift, shift + 8), cut at the sort's last bit (dmask).
+__device__ __forceinline__ uint32_t digit_of(uint32_t key, int shift, uint32_t dmask) { return (key >> shift) & dmask; }
+
+// Digit counts of each tile, digit-major.
+__global__ void __launch_bounds__(kSortBlock) k_dsort_hist(const uint32_t* __restrict__ keys,
+                                                         const int64_t* __restrict__ sizes, int shift, uint32_t dmask,
+                                                         int32_t* __restrict__ hist)
+{
+    __shared__ int32_t s_hist[256];
+    const int64_t n = sizes[0], nt = sizes[1];
+    const int64_t b = blockIdx.x;
+    if (b >= nt) return;
+    s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int r = 0; r < kSortRounds; r++) {
+        const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
+        if (i < n) atomicAdd(&s_hist[digit_of(keys[i], shift, dmask)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nt + b] = s_hist[threadIdx.x];
+}
+
+// Places each pair of the tile: scanned table entry of (digit, tile) + pairs of that digit
+// in the tile's earlier rounds + in earlier waves of this round + earlier lanes of this wave.
+__global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ vals,
+                                                            const int64_t* __restrict__ sizes, int shift,
+                                                            uint32_t dmask, const int32_t* __restrict__ offs,
+                                                            uint32_t* __restrict__ keys_out,
+                                                            uint32_t* __restrict__ vals_out)
+{
+    __shared__ int32_t s_run[256];
+    __shared__ int32_t s_wcnt[kWaves][256];
+    const int64_t n = sizes[0], nt = sizes[1];
+    const int64_t b = blockIdx.x;
+    if (b >= nt) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1;
+    s_run[threadIdx.x] = 0;
+    for (int r = 0; r < kSortRounds; r++) {
+        for (int k = 0; k < kWaves; k++) s_wcnt[k][threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t key = 0, val = 0;
+        if (valid) {
+            key = keys[i];
+            val = vals[i];
+        }
+        const uint32_t d = digit_of(key, shift, dmask);
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            const bool set = (d >> bit) & 1u;
+            const unsigned long long m = __ballot(set);
+            peers &= set ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) s_wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int before = s_run[d];
+            for (int k = 0; k < w; k++) before += s_wcnt[k][d];
+            const int64_t pos = (int64_t)offs[(int64_t)d * nt + b] + before + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int k = 0; k < kWaves; k++) add += s_wcnt[k][threadIdx.x];
+        s_run[threadIdx.x] += add;
+    }
+}
+
+}  // namespace
+
+void dev_scan_i32(vt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* d_n, int64_t n_max, int64_t* d_total,
+                  const char* part_key)
+{
+    const int64_t nb_max = std::max<int64_t>(1, (n_max + kScanChunk - 1) / kScanChunk);
+    int64_t* part = (int64_t*)scratch(ctx, part_key, 8 * (size_t)nb_max);
+    VT_LAUNCH(ctx, "dscan_reduce", k_dscan_reduce, (unsigned)nb_max, kScanBlock, 0, in, d_n, part);
+    VT_LAUNCH(ctx, "dscan_parts", k_dscan_parts, 1, kPartBlock, 0, d_n, part, d_total);
+    VT_LAUNCH(ctx, "dscan_down", k_dscan_down, (unsigned)nb_max, kScanBlock, 0, in, d_n, (const int64_t*)part, out);
+}
+
+bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                        const int64_t* d_count, int64_t mult, int64_t n_max, int bits)
+{
+    if (n_max >= (int64_t(1) << 31))
+        fail(ctx, VT_ERR_LIMIT, "device sort of up to %lld pairs: positions must fit 31 bits", (long long)n_max);
+    const int64_t nt_max = std::max<int64_t>(1, (n_max + kTile - 1) / kTile);
+    int64_t* sizes = (int64_t*)scratch(ctx, "dsort_sizes", 8 * 4);
+    int32_t* hist = (int32_t*)scratch(ctx, "dsort_hist", 4 * 256 * (size_t)nt_max);
+    int32_t* offs = (int32_t*)scratch(ctx, "dsort_offs", 4 * 256 * (size_t)nt_max);
+    VT_LAUNCH(ctx, "dsort_sizes", k_dsort_sizes, 1, 64, 0, d_count, mult, sizes);
+    bool alt = false;
+    for (int shift = 0; shift < bits; shift += 8) {
+        const uint32_t dmask = bits - shift >= 8 ? 255u : (1u << (bits - shift)) - 1u;
+        uint32_t* ki = alt ? keys_alt : keys;
+        uint32_t* vi = alt ? vals_alt : vals;
+        uint32_t* ko = alt ? keys : keys_alt;
+        uint32_t* vo = alt ? vals : vals_alt;
+        VT_LAUNCH(ctx, "dsort_hist", k_dsort_hist, (unsigned)nt_max, kSortBlock, 0, (const uint32_t*)ki,
+                  (const int64_t*)sizes, shift, dmask, hist);
+        dev_scan_i32(ctx, hist, offs, sizes + 2, 256 * nt_max, nullptr, "dsort_part");
+        VT_LAUNCH(ctx, "dsort_scatter", k_dsort_scatter, (unsigned)nt_max, kSortBlock, 0, (const uint32_t*)ki,
+                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, (const int32_t*)offs, ko, vo);
+        alt = !alt;
+    }
+    return alt;
+}
+
+}  // namespace vt
+
+using namespace vt;
+
+extern "C" {
+
+int vt_dev_sort_pairs(vt_ctx* ctx, int64_t n, int64_t capacity, int32_t bits, const uint32_t* keys,
+                      const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out)
+{
+    if (!ctx) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (n < 0 || capacity < n || bits < 1 || bits > 32 || (n > 0 && (!keys || !vals || !keys_out || !vals_out)))
+            fail(ctx, VT_ERR_ARG, "vt_dev_sort_pairs: bad arguments (n=%lld capacity=%lld bits=%d)", (long long)n,
+                 (long long)capacity, bits);
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        const size_t c1 = (size_t)std::max<int64_t>(capacity, 1);
+        DBuf<uint32_t> k0(ctx, c1), v0(ctx, c1), k1(ctx, c1), v1(ctx, c1);
+        DBuf<int64_t> dn(ctx, 1);
+        h2d(ctx, k0.p, keys, 4 * (size_t)n);
+        h2d(ctx, v0.p, vals, 4 * (size_t)n);
+        h2d(ctx, dn.p, &n, 8);
+        const bool alt = dev_sort_pairs_u32(ctx, k0.p, v0.p, k1.p, v1.p, dn.p, 1, capacity, bits);
+        d2h(ctx, keys_out, alt ? k1.p : k0.p, 4 * (size_t)n);
+        d2h(ctx, vals_out, alt ? v1.p : v0.p, 4 * (size_t)n);
+        sync(ctx);
+    });
+}
+
+int vt_dev_scan(vt_ctx* ctx, int64_t n, int64_t capacity, const int32_t* in, int32_t* out, int64_t* total_out)
+{
+    if (!ctx) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (n < 0 || capacity < n || (n > 0 && (!in || !out)))
+            fail(ctx, VT_ERR_ARG, "vt_dev_scan: bad arguments (n=%lld capacity=%lld)", (long long)n,
+                 (long long)capacity);
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        const size_t c1 = (size_t)std::max<int64_t>(capacity, 1);
+        DBuf<int32_t> a(ctx, c1), b(ctx, c1);
+        DBuf<int64_t> dn(ctx, 2);
+        h2d(ctx, a.p, in, 4 * (size_t)n);
+        h2d(ctx, dn.p, &n, 8);
+        dev_scan_i32(ctx, a.p, b.p, dn.p, capacity, dn.p + 1, "dscan_test_part");
+        d2h(ctx, out, b.p, 4 * (size_t)n);
+        if (total_out) d2h(ctx, total_out, dn.p + 1, 8);
+        sync(ctx);
+    });
+}
+
+}  // extern "C"

@@ -222,6 +222,17 @@ void sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, int64_t n, int 
                     uint32_t* keys_alt, uint32_t* vals_alt, bool* result_in_alt);
 void sort_pairs_u64(vt_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits,
                     uint64_t* keys_alt, uint32_t* vals_alt, bool* result_in_alt);
+// The same scan and sort with the element count in device memory (vt_dsort.hip): grids are
+// sized for n_max and blocks past the count exit, so a sequence of them can be captured in a
+// HIP graph and replayed for any count up to n_max.  Their scratch is allocated on the first
+// call of each size (which must not be under capture).
+//   dev_scan_i32: out = exclusive scan of in[0, *d_n); *d_total (if not null) = the sum.
+//   dev_sort_pairs_u32: stable sort of the first (*d_count) * mult pairs by key bits
+//   [0, bits); returns whether the result is in the alt buffers (fixed by bits).
+void dev_scan_i32(vt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* d_n, int64_t n_max, int64_t* d_total,
+                  const char* part_key);
+bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                        const int64_t* d_count, int64_t mult, int64_t n_max, int bits);
 
 // ---- graph helpers ---------------------------------------------------------------------
 vt_graph* graph_new(vt_ctx* ctx, int64_t nv, int64_t ne);

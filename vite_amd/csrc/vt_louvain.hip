@@ -348,6 +348,7 @@ constexpr bool kSplitEdges = VT_SPLIT_EDGES;
 #endif
 constexpr bool kDppArgmax = VT_DPP_ARGMAX;
 
+
 // (gain, community) argmax step: take the other candidate if its gain is larger, or equal
 // with a smaller id (the reference's order-independent rule, louvain.cpp:2246-2273).
 __device__ __forceinline__ void argmax_take(double& bg, int32_t& bc, double og, int32_t oc)
@@ -380,6 +381,49 @@ __device__ __forceinline__ void group_argmax_dpp(double& bg, int32_t& bc)
     if constexpr (G >= 16) argmax_dpp_step<0x118, 0xF>(bg, bc);  // row_shr:8
     if constexpr (G >= 32) argmax_dpp_step<0x142, 0xA>(bg, bc);  // row_bcast15 into rows 1, 3
     if constexpr (G >= 64) argmax_dpp_step<0x143, 0xC>(bg, bc);  // row_bcast31 into rows 2, 3
+}
+
+#ifndef VT_ARGMAX_MAXMIN
+#define VT_ARGMAX_MAXMIN 1
+#endif
+constexpr bool kArgmaxMaxMin = VT_ARGMAX_MAXMIN;
+
+// All-reduce step inside 16-lane rows: every lane reads the lane CTRL selects in its own row
+// (quad_perm, half-row mirror, row mirror), so no lane sees an identity.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_f64(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_row_i32(int32_t x)
+{
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+
+// The same argmax for groups of 4, 8 or 16 lanes in two all-reductions: the largest gain
+// over the group, then the smallest community id among the lanes holding exactly that gain
+// (the lexicographic rule argmax_take applies, done as max then min).  Every lane of the
+// group ends with the result.  Lane steps: quad_perm [1,0,3,2] (0xB1), quad_perm [2,3,0,1]
+// (0x4E), half-row mirror (0x141), row mirror (0x140).
+template <int G>
+__device__ __forceinline__ void group_argmax_maxmin(double& bg, int32_t& bc)
+{
+    static_assert(G == 4 || G == 8 || G == 16, "groups inside one 16-lane row");
+    double m = bg;
+    m = fmax(m, dpp_row_f64<0xB1>(m));
+    m = fmax(m, dpp_row_f64<0x4E>(m));
+    if constexpr (G >= 8) m = fmax(m, dpp_row_f64<0x141>(m));
+    if constexpr (G >= 16) m = fmax(m, dpp_row_f64<0x140>(m));
+    int32_t c = (bg == m) ? bc : INT_MAX;
+    c = min(c, dpp_row_i32<0xB1>(c));
+    c = min(c, dpp_row_i32<0x4E>(c));
+    if constexpr (G >= 8) c = min(c, dpp_row_i32<0x141>(c));
+    if constexpr (G >= 16) c = min(c, dpp_row_i32<0x140>(c));
+    bg = m;
+    bc = c;
 }
 
 // Table of one group (G lanes, one vertex at a time) in LDS.
@@ -654,7 +698,9 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                 bc = take ? ck : bc;
             }
             constexpr int kWriter = kDppArgmax ? G - 1 : 0;
-            if constexpr (kDppArgmax) {
+            if constexpr (kArgmaxMaxMin && G <= 16) {
+                group_argmax_maxmin<G>(bg, bc);
+            } else if constexpr (kDppArgmax) {
                 group_argmax_dpp<G>(bg, bc);
             } else {
                 for (int o = G / 2; o > 0; o >>= 1) {
@@ -1604,11 +1650,12 @@ __global__ void k_batch_flags(int64_t nv, const int64_t* __restrict__ target, co
 
 // The end of the reference loop body per graph (louvain.cpp:540-577): currMod from the
 // sums, leave when it gained less than thresh, else prevMod = max(currMod, lower) and rotate.
+// status[0] counts the graphs that go on and status[1] their vertices (voff: vertex ranges).
 __global__ void k_batch_decide(int32_t ng, const double* __restrict__ sums, const double* __restrict__ constant,
                                const double* __restrict__ thresh, const double* __restrict__ lower,
                                unsigned char* __restrict__ gact, unsigned char* __restrict__ grot,
                                double* __restrict__ prev, int32_t* __restrict__ iters, int32_t max_iters,
-                               int32_t* __restrict__ nactive)
+                               const int64_t* __restrict__ voff, unsigned long long* __restrict__ status)
 {
     for (int32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
         grot[g] = 0;
@@ -1626,7 +1673,8 @@ __global__ void k_batch_decide(int32_t ng, const double* __restrict__ sums, cons
             continue;
         }
         grot[g] = 1;
-        atomicAdd(nactive, 1);
+        atomicAdd(status, 1ull);
+        atomicAdd(status + 1, (unsigned long long)(voff[g + 1] - voff[g]));
     }
 }
 
@@ -2213,7 +2261,10 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     ph.alloc(false);
     DBuf<double> gconst(ctx, ng), gsums(ctx, 2 * (size_t)ng), gprev(ctx, ng), gthr(ctx, ng), glow(ctx, ng);
     DBuf<unsigned char> gact(ctx, ng), grot(ctx, ng);
-    DBuf<int32_t> giters(ctx, ng), nact(ctx, 1);
+    DBuf<int32_t> giters(ctx, ng);
+    // [0] graphs still iterating, [1] their vertices (set by each round's decisions);
+    // [2] nv, the length of the movers' flag scan; [3] movers (that scan's total)
+    DBuf<unsigned long long> status(ctx, 4);
     ph.gid = d_gid;
     ph.gact = gact.p;
     ph.gconst = gconst.p;
@@ -2231,6 +2282,8 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     {
         std::vector<unsigned char> ones((size_t)ng, 1);
         h2d(ctx, gact.p, ones.data(), (size_t)ng);
+        const unsigned long long st0[4] = {(unsigned long long)ng, (unsigned long long)nv, (unsigned long long)nv, 0};
+        h2d(ctx, status.p, st0, sizeof(st0));
     }
     ph.build_lists(nullptr, 1, false);
 
@@ -2243,83 +2296,45 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     int8_t* dr = (int8_t*)scratch(ctx, "bev_s", (size_t)nev);
     int32_t* mflags = (int32_t*)scratch(ctx, "bmv_flags", 4 * (size_t)std::max<int64_t>(nv, 1));
     int32_t* mpos = (int32_t*)scratch(ctx, "bmv_pos", 4 * (size_t)std::max<int64_t>(nv, 1));
-    DBuf<int64_t> dmov(ctx, 1);
     const uint32_t none = (uint32_t)nv;
     const int bits = (int)bits_for(std::max<int64_t>(nv, 1));
-    // which buffer pair the folds read the sorted events from: fixed once the folds are
-    // captured into a graph, and a sort that ends in the other pair is copied over
-    bool exp_alt = false, exp_fixed = false;
+    int64_t* d_nv = (int64_t*)(status.p + 2);
+    int64_t* d_movers = (int64_t*)(status.p + 3);
 
-    auto pre = [&]() {
+    // One round of the reference loop body (louvain.cpp:498-577) for every graph still
+    // iterating, with no host read: the sweep; the movers' flags and their scan (whose total
+    // is the mover count, left on the device); two events per mover, sorted stably by
+    // community with the device-count sort; the per-community delta folds; the per-graph
+    // sums, decisions and rotation.  A graph that has left the phase is skipped by every
+    // step, so a round after the last graph has left changes nothing.
+    auto round = [&]() {
         ph.sweep(ph.sweeps[0]);
         VT_LAUNCH(ctx, "batch_flags", k_batch_flags, grid_for(nv, 256, 8192), 256, 0, nv, (const int64_t*)ph.target.p,
                   (const int64_t*)ph.curr.p, d_gid, (const unsigned char*)gact.p, mflags);
-    };
-    // between the graphs: the movers' events compacted and sorted (the library scan and
-    // sort stay outside capture; the sort's size is the mover count, read back here, and in
-    // the same wait the count of graphs the last round left iterating).  Returns false when
-    // none was left: the sweep just run skipped every vertex, and the phase is over.
-    int32_t active = ng, last_active = ng;
-    // the sweep lists drop the vertices of finished graphs once those are half the listed ones
-    std::vector<int64_t> hvoff((size_t)ng + 1);
-    d2h(ctx, hvoff.data(), seg.voff.p, 8 * ((size_t)ng + 1));
-    int64_t listed = nv;
-    bool relist = false;
-    int relists = 0;
-    auto sort = [&]() -> bool {
-        exclusive_scan_i32(ctx, mflags, mpos, nv);
-        VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)mflags, (const int32_t*)mpos, nv, dmov.p);
-        int64_t movers = 0;
-        d2h(ctx, &movers, dmov.p, 8);
-        d2h(ctx, &active, nact.p, 4);
-        if (active == 0) return false;
-        if (active < last_active) {
-            last_active = active;
-            std::vector<unsigned char> ga((size_t)ng);
-            d2h(ctx, ga.data(), gact.p, (size_t)ng);
-            int64_t live = 0;
-            for (int32_t k = 0; k < ng; k++)
-                if (ga[k]) live += hvoff[k + 1] - hvoff[k];
-            if (2 * live < listed) {
-                relist = true;
-                listed = live;
-            }
-        }
-        if (movers == 0) return true;
+        dev_scan_i32(ctx, mflags, mpos, d_nv, nv, d_movers, "bmv_part");
         VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nv, 256, 8192), 256, 0, (const int32_t*)nullptr, nv,
                   (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p, (const int64_t*)ph.curr.p,
                   k0, v0);
-        bool alt = false;
-        sort_pairs_u32(ctx, k0, v0, 2 * movers, bits, k1, v1, &alt);
-        if (!exp_fixed) {
-            exp_alt = alt;
-        } else if (alt != exp_alt) {
-            d2d(ctx, exp_alt ? k1 : k0, alt ? k1 : k0, 4 * 2 * (size_t)movers);
-            d2d(ctx, exp_alt ? v1 : v0, alt ? v1 : v0, 4 * 2 * (size_t)movers);
-        }
-        return true;
-    };
-    auto post = [&]() {
-        ph.fold_events(exp_alt ? k1 : k0, exp_alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p, dmov.p);
+        const bool alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, d_movers, 2, nev, bits);
+        ph.fold_events(alt ? k1 : k0, alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p, d_movers);
         if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
         else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
-        dzero(ctx, nact.p, 4);
+        dzero(ctx, status.p, 16);
         VT_LAUNCH(ctx, "batch_decide", k_batch_decide, grid_for(ng, 256, 64), 256, 0, ng, (const double*)gsums.p,
                   (const double*)gconst.p, (const double*)gthr.p, (const double*)glow.p, gact.p, grot.p, gprev.p,
-                  giters.p, 0, nact.p);
+                  giters.p, 0, (const int64_t*)seg.voff.p, status.p);
         VT_LAUNCH(ctx, "batch_rotate", k_batch_rotate, grid_for(nv, 256, 8192), 256, 0, nv, d_gid,
                   (const unsigned char*)grot.p, ph.past.p, ph.curr.p, (const int64_t*)ph.target.p);
     };
 
-    int rounds = 0;
-    hipGraphExec_t gpre = nullptr, gpost = nullptr;
-    auto capture = [&](auto&& body, hipGraphExec_t* exec) {
+    hipGraphExec_t gexec = nullptr;
+    auto capture = [&]() {
         const bool prof = ctx->profiling;
         ctx->profiling = false;  // no timing events inside a capture
         hipGraph_t graph = nullptr;
         VT_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-            body();
+            round();
         } catch (...) {
             (void)hipStreamEndCapture(ctx->stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -2328,43 +2343,62 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
         }
         VT_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
         ctx->profiling = prof;
-        const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         VT_HIP(ctx, e);
     };
-    auto destroy = [&]() {
-        if (gpre) (void)hipGraphExecDestroy(gpre);
-        if (gpost) (void)hipGraphExecDestroy(gpost);
-        gpre = gpost = nullptr;
+    // The host reads each round's status from pinned memory one round late: round r is
+    // already queued when the host waits for round r-1's copy, so the device never waits for
+    // the host.  The round after the last graph has left runs as a no-op.
+    unsigned long long* pin = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto release = [&]() {
+        (void)hipStreamSynchronize(ctx->stream);  // the last queued round has finished
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
     };
-    h2d(ctx, nact.p, &active, 4);  // every graph iterates in the first round
+    int rounds = 0, relists = 0;
     try {
-        for (;;) {
-            if (use_graph && rounds >= 1 && !gpre) capture(pre, &gpre);
-            if (use_graph && rounds >= 1 && !gpost) {
-                capture(post, &gpost);
-                exp_fixed = true;
+        VT_HIP(ctx, hipHostMalloc((void**)&pin, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+        for (auto& e : ev) VT_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        int64_t listed = nv;
+        for (int r = 0;; r++) {
+            // round 0 runs directly: it allocates the scratch the captured rounds reuse
+            if (use_graph && r >= 1 && !gexec) capture();
+            if (gexec) VT_HIP(ctx, hipGraphLaunch(gexec, ctx->stream));
+            else round();
+            unsigned long long* slot = pin + 2 * (r & 1);
+            VT_HIP(ctx, hipMemcpyAsync(slot, status.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            VT_HIP(ctx, hipEventRecord(ev[r & 1], ctx->stream));
+            if (r == 0) continue;
+            VT_HIP(ctx, hipEventSynchronize(ev[(r - 1) & 1]));
+            const unsigned long long* prev = pin + 2 * ((r - 1) & 1);
+            const int64_t active = (int64_t)prev[0], live = (int64_t)prev[1];
+            if (active == 0) {
+                rounds = r;  // rounds 0 .. r-1 had graphs iterating; round r was a no-op
+                break;
             }
-            if (gpre) VT_HIP(ctx, hipGraphLaunch(gpre, ctx->stream));
-            else pre();
-            if (!sort()) break;
-            if (gpost) VT_HIP(ctx, hipGraphLaunch(gpost, ctx->stream));
-            else post();
-            rounds++;
-            if (relist) {
-                // new lists (after this round's decisions): the captured sweep is re-captured
-                relist = false;
+            if (2 * live < listed) {
+                // drop the finished graphs' vertices from the sweep lists (rebuilt after the
+                // queued round, from the flags as they stand then); the round is captured anew
+                listed = live;
                 relists++;
                 ph.build_lists(nullptr, 1, false);
-                if (gpre) (void)hipGraphExecDestroy(gpre);
-                gpre = nullptr;
+                sync(ctx);  // the queued round, which replayed the old capture, has finished
+                if (gexec) (void)hipGraphExecDestroy(gexec);
+                gexec = nullptr;
             }
         }
     } catch (...) {
-        destroy();
+        release();
         throw;
     }
-    destroy();
+    release();
     if (!u->part) u->part = (int64_t*)dalloc(ctx, 8 * (size_t)std::max<int64_t>(nv, 1));
     d2d(ctx, u->part, ph.past.p, 8 * (size_t)nv);
     q_out.assign((size_t)ng, 0.0);

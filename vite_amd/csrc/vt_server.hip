@@ -1,15 +1,16 @@
 // Serving form: a request loop that runs community-detection jobs (generate a seeded graph
 // on the device, then the whole phase loop of main.cpp:242-513) on several slots at once.
 // Each slot is a worker thread with its own context, hence its own HIP stream, so the jobs
-// of different slots overlap on the device.  Requests are admitted strictly in arrival
-// order, as soon as a slot is free and the request's HBM reservation fits beside the
-// reservations of the running jobs; a job cannot allocate beyond its reservation (its
-// context budget), so the running jobs together never exceed the server's budget.
+// of different slots overlap on the device.  The oldest queued request is admitted first,
+// as soon as a slot is free and its HBM reservation fits beside the reservations of the
+// running jobs; a job cannot allocate beyond its reservation (its context budget), so the
+// running jobs together never exceed the server's budget.
 //
 // Small requests are batched: a slot that admits a small plain-Louvain request also takes
-// the small requests with the same run parameters queued right behind it (up to a batch
-// size, and while their reservation fits) and runs them as one batched job (vt_batch_*),
-// whose per-graph results equal vt_run's on each graph alone.
+// the other queued small requests with the same run parameters, oldest first, wherever
+// they wait in the queue (up to a batch size, and while their reservation fits), and runs
+// them as one batched job (vt_batch_*), whose per-graph results equal vt_run's on each
+// graph alone.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -185,14 +186,19 @@ void worker(vt_server* s, int slot)
             grp.push_back(j);
             reserve = j->reserve;
             if (batchable(s, j->req)) {
-                // the requests right behind it, while they match and fit: a batched job holds
-                // its members' graphs and their union, so it reserves twice their sum
+                // the queued requests that match, oldest first, while they fit: a batched job
+                // holds its members' graphs and their union, so it reserves twice their sum.
+                // Requests that do not match keep their places, so a large request ahead of
+                // small ones no longer holds them back from this job.
                 uint64_t sum = j->reserve;
-                while (!s->queue.empty() && (int32_t)grp.size() < s->max_batch) {
-                    Job* k = s->queue.front();
-                    if (!batchable(s, k->req) || !same_run(k->req.run, j->req.run)) break;
+                for (auto it = s->queue.begin(); it != s->queue.end() && (int32_t)grp.size() < s->max_batch;) {
+                    Job* k = *it;
+                    if (!batchable(s, k->req) || !same_run(k->req.run, j->req.run)) {
+                        ++it;
+                        continue;
+                    }
                     if (s->reserved + 2 * (sum + k->reserve) > s->budget) break;
-                    s->queue.pop_front();
+                    it = s->queue.erase(it);
                     grp.push_back(k);
                     sum += k->reserve;
                 }
