@@ -162,6 +162,17 @@ int vt_sweep(vt_ctx* ctx, const vt_graph* g, const int64_t* curr, const int64_t*
 int vt_modularity(vt_ctx* ctx, int64_t nv, const double* cw, const double* cdeg, double constant,
                   int32_t flags, double* q_out);
 
+/* vt_modularity on DEVICE arrays, allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  d_cw and d_cdeg hold nv doubles each and d_q_out one double, all device
+ * memory of ctx's device; the result has the same bits as vt_modularity's on the same
+ * values and flags.  The partial sums live in a fixed 16.25 KiB device array per device
+ * (any nv fits: the tree has at most 1024 blocks), resolved when a context is created on
+ * that device and held until the process ends.  Every call on one device shares that
+ * array, so calls that overlap on one device (from several contexts or threads) race.
+ * Synchronous at return like every other entry point. */
+int vt_modularity_noalloc(vt_ctx* ctx, int64_t nv, const double* d_cw, const double* d_cdeg, double constant,
+                          int32_t flags, double* d_q_out);
 
 /* Vertex degrees (row left folds) and 1/2m. */
 int vt_init_degrees(vt_ctx* ctx, const vt_graph* g, int32_t flags, double* vdeg_out,

@@ -122,6 +122,8 @@ _SIGS = {
     "vt_sweep": (C.c_int, [C.c_void_p, C.c_void_p, _i64p, _i64p, _f64p, _f64p, C.c_double, _i64p, _f64p,
                            _i64p, _f64p]),
     "vt_modularity": (C.c_int, [C.c_void_p, C.c_int64, _f64p, _f64p, C.c_double, C.c_int32, _f64p]),
+    "vt_modularity_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
+                                        C.c_void_p]),
     "vt_dev_sort_pairs": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "vt_dev_scan": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]),

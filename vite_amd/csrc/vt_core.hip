@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -542,6 +543,52 @@ void graph_finalize(vt_ctx* ctx, vt_graph* g)
 using namespace vt;
 
 // ======================================================================================
+// Allocation-free modularity (vt_modularity_noalloc): its partial sums live in a fixed
+// device array instead of memory the context allocates.
+// ======================================================================================
+
+// The tree's block partials (one double2 per block, at most kSumGrid blocks: 16 KiB) and
+// the two final sums.  Every device holds its own copy until the process ends.  Calls on
+// one device share that copy, so two calls that overlap on one device race.
+constexpr size_t kNoAllocPartBytes = sizeof(double2) * kSumGrid;
+constexpr size_t kNoAllocBytes = kNoAllocPartBytes + 256;
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};  // device address of each device's copy
+
+// Resolves the current device's copy of noalloc_scratch, once per device.  In host code the
+// array's name is only a stand-in for the symbol; kernels get this address instead.
+hipError_t noalloc_setup(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e == hipSuccess) g_noalloc_ptr[device] = p;
+    return e;
+}
+
+void* noalloc_ptr(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
+// k_modularity with 1/2m passed by value: |(Σcw)c − (Σa²)c·c|, the same expression.
+__global__ void k_modularity_value(const double* __restrict__ sums, double constant, double* __restrict__ q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = fabs((sums[0] * constant) - (sums[1] * constant * constant));
+}
+
+}  // namespace
+
+// ======================================================================================
 // C ABI: context and graphs
 // ======================================================================================
 
@@ -555,6 +602,7 @@ int vt_ctx_create(int device, uint64_t hbm_budget_bytes, vt_ctx** out)
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return VT_ERR_HIP;
     if (device < 0 || device >= n) return VT_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return VT_ERR_HIP;
+    if (noalloc_setup(device) != hipSuccess) return VT_ERR_HIP;
     vt_ctx* c = new vt_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
@@ -691,6 +739,41 @@ int vt_graph_free(vt_ctx* ctx, vt_graph* g)
         VT_HIP(ctx, hipSetDevice(ctx->device));
         sync(ctx);
         graph_delete(ctx, g);
+    });
+}
+
+int vt_modularity_noalloc(vt_ctx* ctx, int64_t nv, const double* d_cw, const double* d_cdeg, double constant,
+                          int32_t flags, double* d_q_out)
+{
+    if (!ctx) return VT_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (nv < 0 || !d_q_out || (nv > 0 && (!d_cw || !d_cdeg)))
+            fail(ctx, VT_ERR_ARG, "vt_modularity_noalloc: bad arguments (nv=%lld)", (long long)nv);
+        VT_HIP(ctx, hipSetDevice(ctx->device));
+        unsigned char* base = (unsigned char*)noalloc_ptr(ctx->device);
+        if (!base) fail(ctx, VT_ERR_STATE, "vt_modularity_noalloc: no scratch address for device %d", ctx->device);
+        int64_t grid = 0, chunk = 0;
+        sum2_plan(nv, &grid, &chunk);
+        if ((size_t)grid * sizeof(double2) > kNoAllocPartBytes)
+            fail(ctx, VT_ERR_LIMIT, "vt_modularity_noalloc: %lld partial sums exceed the %zu-byte scratch",
+                 (long long)grid, kNoAllocPartBytes);
+        double2* part = (double2*)base;
+        double* sums = (double*)(base + kNoAllocPartBytes);
+        // the same kernels, grid and chunk as vt_modularity, so the same bits
+        if (flags & VT_FLAG_SEQUENTIAL_SUMS) {
+            hipLaunchKernelGGL(k_seq_sum2, dim3(1), dim3(64), 0, ctx->stream, d_cw, d_cdeg, nv, sums);
+        } else {
+            hipLaunchKernelGGL(k_sum2_partial, dim3((unsigned)grid), dim3(kSumBlock), 0, ctx->stream, d_cw, d_cdeg, nv,
+                               chunk, part);
+            VT_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_sum2_final, dim3(1), dim3(kSumBlock), 0, ctx->stream, (const double2*)part, (int)grid,
+                               sums);
+        }
+        VT_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_modularity_value, dim3(1), dim3(64), 0, ctx->stream, (const double*)sums, constant,
+                           d_q_out);
+        VT_HIP(ctx, hipGetLastError());
+        VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     });
 }
 
