@@ -34,7 +34,12 @@ HBM_PEAK_GBS = 8000.0
 # launch named "local_move", the dominant one on random geometric graphs; buckets 0, 1
 # and 3 run in "local_move_rest" and larger degree classes in their own launches.
 G16_BUCKET = 2
-G16_KERNEL = "k_local_move<false, 2>"
+
+
+def g16_kernel(tuning):
+    """The bucket-2 launch's kernel as rocprof names it: single graph, part 2, and whether it
+    reads the per-vertex community records (the vertex_records tuning)."""
+    return f"k_local_move<false, 2, {'true' if tuning.get('vertex_records') else 'false'}>"
 # rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) against the bytes a plain 1 GiB
 # device copy really moves: FETCH_SIZE reads half of them, WRITE_SIZE all of them.
 FETCH_SCALE = 2.0
@@ -647,7 +652,7 @@ def main():
         f"({Vf} of {V} rows, {Ef} of {E} entries), {achieved:.1f} GB/s algorithmic "
         f"({100 * achieved / HBM_PEAK_GBS:.1f}% of peak), {value / 1e9:.2f} G edges/s")
     tag = f"rgg_n{V}"
-    traffic, traffic_src = measured_traffic("local_move", tag, G16_KERNEL)
+    traffic, traffic_src = measured_traffic("local_move", tag, g16_kernel(tuning))
 
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -659,7 +664,7 @@ def main():
                    "threshold": 1e-3, "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": G16_KERNEL, "kernel_avg_ms": kavg, "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel": g16_kernel(tuning), "kernel_avg_ms": kavg, "algorithmic_bytes_per_launch": alg_bytes,
                      "bytes_formula": "24E + 56V over the kernel's rows (bucket 2: 16-lane groups)",
                      "sweep_achieved": (24 * E + 56 * V) / (ms_per_step * 1e-3) / 1e9,
                      "sweep_formula": "24E + 56V over all rows / wall time of the whole sweep (every launch)",

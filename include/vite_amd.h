@@ -71,7 +71,7 @@ int vt_ctx_reset_stats(vt_ctx* ctx);
  * 2 x lanes - 1) or one 64-lane wave with a 2048-slot table (b = 5, at most 1023); longer rows
  * use tables in HBM.  blocks_per_bucket caps the workgroups one bucket gets per launch.
  * Any valid setting gives bit-identical results; only the speed changes.
- * Default {7, 15, 31, 63, 127, 1023}, 16384. */
+ * Default {7, 15, 31, 63, 127, 1023}, 16384, {255, 511}, 0. */
 typedef struct {
     int32_t bucket_max_degree[6];
     int32_t blocks_per_bucket;
@@ -79,6 +79,12 @@ typedef struct {
      * [0] get 512 LDS slots, up to [1] 1024, the others 2048.  A table stays at most 3/4
      * full, so [0] <= 383 and [1] <= 767 (-1: none); [0] <= [1]. */
     int32_t wide_table_max_degree[2];
+    /* 1: the group kernels (rows up to bucket 4) read each neighbour's community, and each
+     * candidate community's size and degree, from one 16-byte record per vertex id, rebuilt
+     * before every sweep; a candidate community whose id is the neighbour's own id then comes
+     * with the neighbour's line.  Pays on graphs whose neighbours are scattered (random
+     * reads); 0 (the default): separate arrays. */
+    int32_t vertex_records;
 } vt_tuning;
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out);
 int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t);

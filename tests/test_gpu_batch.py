@@ -115,6 +115,24 @@ def test_batch_autotune_keeps_results(ctx, mixed):
         _same(a, r)
 
 
+def test_batch_with_community_records(ctx, mixed):
+    """The batched sweeps reading the per-vertex community records (vertex_records = 1, the
+    records of the whole union rebuilt before every captured round) give every graph what
+    it gets alone with separate arrays."""
+    host, dev = mixed
+    default = ctx.tuning
+    alone = [g.run(threshold_cycling=True) for g in dev]
+    b = vite_amd.Batch(ctx, dev)
+    try:
+        ctx.set_tuning(vertex_records=1)
+        res, _ = b.run(threshold_cycling=True)
+    finally:
+        b.free()
+        ctx.set_tuning(**default)
+    for a, r in zip(alone, res):
+        _same(a, r)
+
+
 def test_batch_rejects(ctx):
     g_empty = ctx.upload(vite_amd.CSR(np.zeros(2, np.int64), np.zeros(0, po.EDGE_DTYPE)))
     with pytest.raises(vite_amd.VtError):

@@ -314,7 +314,10 @@ TUNINGS = [dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=16
            dict(bucket_max_degree=[-1, -1, -1, -1, -1, 1023], blocks_per_bucket=256, wide_table_max_degree=[150, 600]),
            # bucket 2 (16-lane groups, a launch of its own) alone, then everything but it
            dict(bucket_max_degree=[-1, -1, 47, -1, 127, 1023], blocks_per_bucket=512),
-           dict(bucket_max_degree=[7, 15, -1, 63, 127, 1023], blocks_per_bucket=512)]
+           dict(bucket_max_degree=[7, 15, -1, 63, 127, 1023], blocks_per_bucket=512),
+           # the group kernels reading the per-vertex community records
+           dict(bucket_max_degree=[7, 15, 47, 63, 191, 1023], blocks_per_bucket=16384, vertex_records=1),
+           dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=8, vertex_records=1)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=[str(i) for i in range(len(TUNINGS))])
@@ -335,6 +338,25 @@ def test_tuning_does_not_change_results(ctx, name, tuning):
         q, cv, it = dg.louvain_phase(sequential=True)
         assert (q, it) == (q_o, it_o)
         assert np.array_equal(cv, cv_o)
+    finally:
+        ctx.set_tuning(**default)
+
+
+def test_tuning_records_in_color_and_et_phases(ctx):
+    """The community records are rebuilt before each color's sweep (the degrees change
+    between colors) and read under early termination: whole phases stay exact."""
+    default = ctx.tuning
+    try:
+        ctx.set_tuning(vertex_records=1)
+        for name in ("rgg", "ba"):
+            g, dg = _upload(ctx, name)
+            nc, colors = po.color(g, 4)
+            for kw in (dict(mode="color", num_color=nc + 1, colors=colors), dict(et_type=2, et_delta=0.25),
+                       dict(mode="vorder", num_color=nc + 1, colors=colors, et_type=1)):
+                q_o, cv_o, it_o = po.louvain_phase(g, **kw)
+                q, cv, it = dg.louvain_phase(sequential=True, **kw)
+                assert (q, it) == (q_o, it_o), (name, kw)
+                assert np.array_equal(cv, cv_o), (name, kw)
     finally:
         ctx.set_tuning(**default)
 

@@ -162,32 +162,51 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_hist(const uint32_t* __res
     hist[(int64_t)threadIdx.x * nt + b] = s_hist[threadIdx.x];
 }
 
-// Places each pair of the tile: scanned table entry of (digit, tile) + pairs of that digit
-// in the tile's earlier rounds + in earlier waves of this round + earlier lanes of this wave.
+// Places the pairs of one tile in two steps.  First every pair gets its place in the tile
+// sorted by digit (the tile's pairs of smaller digits, then those of its digit in earlier
+// rounds, earlier waves of this round and earlier lanes of this wave) and is staged there in
+// LDS.  Then the staged tile is written out in that order: a pair at staged place p with
+// digit d goes to (scanned table entry of (d, tile)) + (p - first staged place of d), so the
+// pairs of one digit leave as one contiguous run and the writes coalesce.
 __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ vals,
                                                             const int64_t* __restrict__ sizes, int shift,
-                                                            uint32_t dmask, const int32_t* __restrict__ offs,
+                                                            uint32_t dmask, const int32_t* __restrict__ hist,
+                                                            const int32_t* __restrict__ offs,
                                                             uint32_t* __restrict__ keys_out,
                                                             uint32_t* __restrict__ vals_out)
 {
-    __shared__ int32_t s_run[256];
-    __shared__ int32_t s_wcnt[kWaves][256];
+    __shared__ int32_t s_run[256];              // digit's pairs placed by earlier rounds
+    __shared__ int32_t s_wcnt[kWaves][256];     // this round's pairs per wave and digit
+    __shared__ int32_t s_first[256];            // first staged place of each digit
+    __shared__ int32_t s_out[256];              // global place of each digit's first pair, minus s_first
+    __shared__ int64_t s_wave[kWaves];
+    __shared__ uint32_t s_key[kTile], s_val[kTile];
     const int64_t n = sizes[0], nt = sizes[1];
     const int64_t b = blockIdx.x;
     if (b >= nt) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1;
-    s_run[threadIdx.x] = 0;
+    {
+        // the tile's digit counts, scanned: first staged place of each digit
+        const int32_t cnt = hist[(int64_t)threadIdx.x * nt + b];
+        int64_t total = 0;
+        const int32_t first = (int32_t)(block_inclusive_scan(cnt, s_wave, &total) - cnt);
+        s_first[threadIdx.x] = first;
+        s_out[threadIdx.x] = offs[(int64_t)threadIdx.x * nt + b] - first;
+        s_run[threadIdx.x] = 0;
+    }
+    const int64_t base = b * kTile;
+    const int tile_n = (int)(n - base < kTile ? n - base : kTile);
     for (int r = 0; r < kSortRounds; r++) {
         for (int k = 0; k < kWaves; k++) s_wcnt[k][threadIdx.x] = 0;
         __syncthreads();
-        const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
-        const bool valid = i < n;
+        const int li = r * kSortBlock + threadIdx.x;
+        const bool valid = li < tile_n;
         uint32_t key = 0, val = 0;
         if (valid) {
-            key = keys[i];
-            val = vals[i];
+            key = keys[base + li];
+            val = vals[base + li];
         }
         const uint32_t d = digit_of(key, shift, dmask);
         // lanes of this wave holding the same digit
@@ -201,16 +220,22 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __
         if (valid && rank == 0) s_wcnt[w][d] = __popcll(peers);
         __syncthreads();
         if (valid) {
-            int before = s_run[d];
-            for (int k = 0; k < w; k++) before += s_wcnt[k][d];
-            const int64_t pos = (int64_t)offs[(int64_t)d * nt + b] + before + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            int place = s_first[d] + s_run[d] + rank;
+            for (int k = 0; k < w; k++) place += s_wcnt[k][d];
+            s_key[place] = key;
+            s_val[place] = val;
         }
         __syncthreads();
         int add = 0;
         for (int k = 0; k < kWaves; k++) add += s_wcnt[k][threadIdx.x];
         s_run[threadIdx.x] += add;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < tile_n; p += kSortBlock) {
+        const uint32_t key = s_key[p];
+        const int64_t pos = (int64_t)s_out[digit_of(key, shift, dmask)] + p;
+        keys_out[pos] = key;
+        vals_out[pos] = s_val[p];
     }
 }
 
@@ -247,7 +272,8 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
                   (const int64_t*)sizes, shift, dmask, hist);
         dev_scan_i32(ctx, hist, offs, sizes + 2, 256 * nt_max, nullptr, "dsort_part");
         VT_LAUNCH(ctx, "dsort_scatter", k_dsort_scatter, (unsigned)nt_max, kSortBlock, 0, (const uint32_t*)ki,
-                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, (const int32_t*)offs, ko, vo);
+                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, (const int32_t*)hist,
+                  (const int32_t*)offs, ko, vo);
         alt = !alt;
     }
     return alt;

@@ -122,6 +122,18 @@ struct Table {
     uint32_t capmask;
 };
 
+// Community state indexed by vertex id, one 16-byte record per id: the vertex's community
+// (currComm) and, for the community whose id is this vertex id, its size and degree
+// (localCinfo).  A neighbour's record holds its community id; when that community is the
+// neighbour's own id (the neighbour founded it, or still is alone), the community's degree
+// comes in the same line, so the candidate's degree read that follows finds it in cache.
+struct __align__(16) VRec {
+    int32_t comm;
+    int32_t size;
+    double deg;
+};
+static_assert(sizeof(VRec) == 16, "record must be 16 bytes");
+
 struct MoveArgs {
     const int64_t* offs;
     const DEdge* edges;
@@ -144,6 +156,9 @@ struct MoveArgs {
     const int32_t* tail32;
     const double* wgt;
     const int32_t* curr32;
+    // Records of the same community state (null unless the launch tuning asks for them); the
+    // group kernels then read communities, sizes and degrees from them.
+    const VRec* rec;
 };
 
 // Inserts or finds community c; returns its slot.
@@ -529,7 +544,7 @@ struct VHeader<true> : VHeaderBase {
     double cst;
 };
 
-template <bool BATCH>
+template <bool BATCH, bool REC>
 __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t v, bool have)
 {
     VHeader<BATCH> h;
@@ -543,7 +558,8 @@ __device__ __forceinline__ VHeader<BATCH> load_header(const MoveArgs& a, int32_t
         const bool live = !(a.active && !a.active[v]);
         h.e0 = a.offs[v];
         h.deg = live ? (int32_t)(a.offs[v + 1] - h.e0) : -1;
-        if constexpr (kSplitEdges) h.cc = a.curr32[v];
+        if constexpr (REC) h.cc = a.rec[v].comm;
+        else if constexpr (kSplitEdges) h.cc = a.curr32[v];
         else h.cc = (int32_t)a.curr[v];
         h.kv = a.vdeg[v];
         if constexpr (BATCH) {
@@ -597,21 +613,22 @@ __device__ __forceinline__ void row_issue_edges(const MoveArgs& a, const VHeader
     }
 }
 
-template <int NCH>
+template <bool REC, int NCH>
 __device__ __forceinline__ void row_issue_gathers(const MoveArgs& a, int32_t v, RowRegs<NCH>& r)
 {
 #pragma unroll
     for (int u = 0; u < NCH; u++) {
         r.c[u] = 0;
         if (r.ok[u]) {
-            if constexpr (kSplitEdges) r.c[u] = a.curr32[r.t[u]];
+            if constexpr (REC) r.c[u] = a.rec[r.t[u]].comm;
+            else if constexpr (kSplitEdges) r.c[u] = a.curr32[r.t[u]];
             else r.c[u] = reinterpret_cast<const int32_t*>(a.curr)[2 * (int64_t)r.t[u]];
         }
         r.self[u] = r.ok[u] && r.t[u] == v;
     }
 }
 
-template <int G, int CAP, int BLOCK, bool PIPE, bool BATCH, int NCH>
+template <int G, int CAP, int BLOCK, bool PIPE, bool BATCH, int NCH, bool REC>
 __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const GroupTable<G>& t, double* ecc_cell,
                                                   const int32_t* __restrict__ list, int64_t n, int64_t idx,
                                                   int64_t stride, int lane, int gbase, unsigned* inactive_local)
@@ -619,12 +636,12 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
     static_assert(CAP >= 4 * G && CAP <= 256, "occ holds 8-bit slot indexes");
     static_assert(NCH * G <= 3 * CAP / 4, "at most 3/4 of the table filled by distinct communities");
     if (idx >= n) return;
-    VHeader<BATCH> h = load_header<BATCH>(a, list[idx], true);
+    VHeader<BATCH> h = load_header<BATCH, REC>(a, list[idx], true);
     int32_t vn = (idx + stride < n) ? list[idx + stride] : 0;
     RowRegs<NCH> r;
     if constexpr (PIPE) row_issue_edges<G>(a, h, lane, r);
-    VHeader<BATCH> hn = load_header<BATCH>(a, vn, idx + stride < n);
-    if constexpr (PIPE) row_issue_gathers(a, h.v, r);
+    VHeader<BATCH> hn = load_header<BATCH, REC>(a, vn, idx + stride < n);
+    if constexpr (PIPE) row_issue_gathers<REC>(a, h.v, r);
     const double constant0 = BATCH ? 0.0 : *a.constant;
     while (idx < n) {
         const bool has_next = PIPE && idx + stride < n;
@@ -636,9 +653,9 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             row_issue_edges<G>(a, hn, lane, rn);
         } else {
             row_issue_edges<G>(a, h, lane, r);
-            row_issue_gathers(a, h.v, r);
+            row_issue_gathers<REC>(a, h.v, r);
         }
-        const VHeader<BATCH> hnn = load_header<BATCH>(a, vnn, i2 < n);
+        const VHeader<BATCH> hnn = load_header<BATCH, REC>(a, vnn, i2 < n);
         double constant = constant0;
         if constexpr (BATCH) constant = h.cst;
 
@@ -661,8 +678,15 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             // the loads of this vertex have been issued: its LDS work and argmax take the
             // SIMD ahead of waves still waiting on memory, so it reaches its next loads sooner
             __builtin_amdgcn_s_setprio(1);
-            const double ccDeg = a.cdeg[cc];
-            const int64_t ccSize = a.csize[cc];
+            double ccDeg;
+            int64_t ccSize;
+            if constexpr (REC) {
+                ccDeg = a.rec[cc].deg;
+                ccSize = a.rec[cc].size;
+            } else {
+                ccDeg = a.cdeg[cc];
+                ccSize = a.csize[cc];
+            }
             if (lane == 0) *ecc_cell = 0.0;
             double selfLoop = 0.0;
             int nocc = 0;
@@ -674,7 +698,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                                   nocc);
             const double ecc = *ecc_cell;
             // the next vertex's neighbour communities, before waiting on this one's gathers
-            if (has_next) row_issue_gathers(a, hn.v, rn);
+            if (has_next) row_issue_gathers<REC>(a, hn.v, rn);
             // distGetMaxIndex (louvain.cpp:2246-2276) over the occupied slots, clearing them
             const double kv = h.kv;
             const double eix = ecc - selfLoop;
@@ -691,7 +715,10 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                 *kp = -1;
                 *vp = -0.0;
                 const bool cand = ok & ((int64_t)ck != cc);
-                const double ay = a.cdeg[cand ? ck : (int32_t)cc];
+                const int32_t ci = cand ? ck : (int32_t)cc;
+                double ay;
+                if constexpr (REC) ay = a.rec[ci].deg;
+                else ay = a.cdeg[ci];
                 const double gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
                 const bool take = cand & (gain > 0.0) & ((gain > bg) | ((gain == bg) & (ck < bc)));
                 bg = take ? gain : bg;
@@ -716,7 +743,11 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
                 int64_t tgt = cc;
                 if (bg > 0.0) {
                     tgt = bc;
-                    if (ccSize == 1 && tgt > cc && a.csize[tgt] == 1) tgt = cc;  // louvain.cpp:2275-2276
+                    if constexpr (REC) {
+                        if (ccSize == 1 && tgt > cc && a.rec[tgt].size == 1) tgt = cc;  // louvain.cpp:2275-2276
+                    } else {
+                        if (ccSize == 1 && tgt > cc && a.csize[tgt] == 1) tgt = cc;
+                    }
                 }
                 a.target[v] = tgt;
                 a.cw[v] = ecc;
@@ -725,7 +756,7 @@ __device__ __forceinline__ void run_two_pipelined(const MoveArgs& a, const Group
             wave_sync();
             __builtin_amdgcn_s_setprio(0);
         }
-        if (has_next && h.deg <= 0) row_issue_gathers(a, hn.v, rn);
+        if (has_next && h.deg <= 0) row_issue_gathers<REC>(a, hn.v, rn);
         h = hn;
         hn = hnn;
         if constexpr (PIPE) r = rn;
@@ -774,7 +805,7 @@ struct GroupLds {
 
 // Runs one degree bucket inside a block: block `blk` of `nblk` owns list entries
 // blk*NG + g, stepping by nblk*NG.  Every thread of the block must call it.
-template <int G, int CAP, int BLOCK, bool TWO, bool BATCH = false, int NCH = 2>
+template <int G, int CAP, int BLOCK, bool TWO, bool BATCH = false, int NCH = 2, bool REC = false>
 __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const int32_t* __restrict__ list, int64_t n,
                                            int64_t blk, int64_t nblk, unsigned* inactive)
 {
@@ -809,11 +840,11 @@ __device__ __forceinline__ void run_bucket(const MoveArgs& a, char* lds, const i
             // contiguous eighth of the list: neighbouring vertices then share one L2.
             const int64_t x = blk % kXcds, per = (n + kXcds - 1) / kXcds;
             const int64_t lo = x * per < n ? x * per : n, hi = lo + per < n ? lo + per : n;
-            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH, NCH>(a, t, ecc_cell, list + lo, hi - lo,
+            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH, NCH, REC>(a, t, ecc_cell, list + lo, hi - lo,
                                                                (blk / kXcds) * (BLOCK / G) + g,
                                                                (nblk / kXcds) * (BLOCK / G), lane, gbase, inactive);
         } else {
-            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH, NCH>(a, t, ecc_cell, list, n, idx, stride, lane, gbase,
+            run_two_pipelined<G, CAP, BLOCK, kPipeRows, BATCH, NCH, REC>(a, t, ecc_cell, list, n, idx, stride, lane, gbase,
                                                                     inactive);
         }
     } else {
@@ -861,7 +892,7 @@ struct SweepPlan {
 
 enum FusedPart { kPartRest = 1, kPartG16 = 2 };
 
-template <bool BATCH, int PART>
+template <bool BATCH, int PART, bool REC>
 __global__ void __launch_bounds__(kFusedBlock) __attribute__((amdgpu_waves_per_eu(8)))
 k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
 {
@@ -877,12 +908,12 @@ k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
     const int64_t blk = bid - plan.bbeg[b], nblk = plan.bbeg[b + 1] - plan.bbeg[b];
     const int32_t* list = lists + plan.off[b];
     if (PART == kPartG16) {
-        run_bucket<16, 64, 256, true, BATCH, kNch[2]>(a, lds, list, plan.n[2], blk, nblk, &s_inactive);
+        run_bucket<16, 64, 256, true, BATCH, kNch[2], REC>(a, lds, list, plan.n[2], blk, nblk, &s_inactive);
     } else {
         switch (b) {
-            case 0: run_bucket<4, 16, 256, true, BATCH, kNch[0]>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
-            case 1: run_bucket<8, VT_CAP_G8, 256, true, BATCH, kNch[1]>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
-            default: run_bucket<32, 128, 256, true, BATCH, kNch[3]>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
+            case 0: run_bucket<4, 16, 256, true, BATCH, kNch[0], REC>(a, lds, list, plan.n[0], blk, nblk, &s_inactive); break;
+            case 1: run_bucket<8, VT_CAP_G8, 256, true, BATCH, kNch[1], REC>(a, lds, list, plan.n[1], blk, nblk, &s_inactive); break;
+            default: run_bucket<32, 128, 256, true, BATCH, kNch[3], REC>(a, lds, list, plan.n[3], blk, nblk, &s_inactive); break;
         }
     }
     __syncthreads();
@@ -890,13 +921,13 @@ k_local_move(MoveArgs a, const int32_t* __restrict__ lists, SweepPlan plan)
 }
 
 // Bucket 4 (64 <= degree <= 127): four 64-lane groups per block, 256-slot tables.
-template <bool BATCH>
+template <bool BATCH, bool REC>
 __global__ void __launch_bounds__(256) k_local_move_mid(MoveArgs a, const int32_t* __restrict__ list, int64_t n)
 {
     __shared__ __align__(16) char lds[kMidLds];
     __shared__ unsigned s_inactive;
     if (threadIdx.x == 0) s_inactive = 0;
-    run_bucket<64, 256, 256, true, BATCH, kNch[4]>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
+    run_bucket<64, 256, 256, true, BATCH, kNch[4], REC>(a, lds, list, n, blockIdx.x, gridDim.x, &s_inactive);
     __syncthreads();
     if (threadIdx.x == 0 && s_inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)s_inactive);
 }
@@ -1627,6 +1658,18 @@ __global__ void k_narrow_ids(const int64_t* __restrict__ src, int64_t n, int32_t
         dst[i] = (int32_t)src[i];
 }
 
+__global__ void k_pack_records(const int64_t* __restrict__ curr, const int64_t* __restrict__ csize,
+                               const double* __restrict__ cdeg, int64_t n, VRec* __restrict__ rec)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        VRec r;
+        r.comm = (int32_t)curr[i];
+        r.size = (int32_t)csize[i];
+        r.deg = cdeg[i];
+        rec[i] = r;
+    }
+}
+
 // ---- batch of disjoint graphs ------------------------------------------------------------
 
 __global__ void k_seg_constant(const double* __restrict__ sums, int32_t ng, double* __restrict__ constant)
@@ -1742,6 +1785,7 @@ struct Phase {
     // narrowed before every sweep (it changes between sweeps).
     DBuf<int32_t> tail32, curr32;
     DBuf<double> wgt;
+    DBuf<VRec> rec;  // community records, rebuilt before every sweep when the tuning asks
 
     Phase(vt_ctx* c, const vt_graph* gr) : ctx(c), g(gr), nv(gr->nv) {}
 
@@ -1758,6 +1802,11 @@ struct Phase {
                           (const DEdge*)g->edges, g->ne, tail32.p, wgt.p);
         }
         if (nv) VT_LAUNCH(ctx, "narrow_curr", k_narrow_ids, grid_for(nv, 256, 8192), 256, 0, (const int64_t*)curr.p, nv, curr32.p);
+        if (ctx->tuning.vertex_records && nv) {
+            if (!rec.p) rec = DBuf<VRec>(ctx, (size_t)nv);
+            VT_LAUNCH(ctx, "pack_records", k_pack_records, grid_for(nv, 256, 8192), 256, 0, (const int64_t*)curr.p,
+                      (const int64_t*)csize.p, (const double*)cdeg.p, nv, rec.p);
+        }
     }
 
     BucketBounds bounds() const
@@ -1970,6 +2019,7 @@ struct Phase {
         a.tail32 = tail32.p;
         a.wgt = wgt.p;
         a.curr32 = curr32.p;
+        a.rec = (ctx->tuning.vertex_records && rec.p) ? rec.p : nullptr;
         return a;
     }
 
@@ -1988,17 +2038,20 @@ struct Phase {
         if (cs.fused_blocks) {
             const int64_t n16 = cs.plan.bbeg[3] - cs.plan.bbeg[2];
             if (cs.fused_blocks > n16) {
-                auto kern = batch ? k_local_move<true, kPartRest> : k_local_move<false, kPartRest>;
+                auto kern = a.rec ? (batch ? k_local_move<true, kPartRest, true> : k_local_move<false, kPartRest, true>)
+                                  : (batch ? k_local_move<true, kPartRest, false> : k_local_move<false, kPartRest, false>);
                 VT_LAUNCH(ctx, "local_move_rest", kern, (unsigned)(cs.fused_blocks - n16), kFusedBlock, 0, a,
                           (const int32_t*)lists.p, cs.plan);
             }
             if (n16) {
-                auto kern = batch ? k_local_move<true, kPartG16> : k_local_move<false, kPartG16>;
+                auto kern = a.rec ? (batch ? k_local_move<true, kPartG16, true> : k_local_move<false, kPartG16, true>)
+                                  : (batch ? k_local_move<true, kPartG16, false> : k_local_move<false, kPartG16, false>);
                 VT_LAUNCH(ctx, "local_move", kern, (unsigned)n16, kFusedBlock, 0, a, (const int32_t*)lists.p, cs.plan);
             }
         }
         if (cs.mid_n) {
-            auto kern = batch ? k_local_move_mid<true> : k_local_move_mid<false>;
+            auto kern = a.rec ? (batch ? k_local_move_mid<true, true> : k_local_move_mid<false, true>)
+                              : (batch ? k_local_move_mid<true, false> : k_local_move_mid<false, false>);
             const int64_t nb = std::min<int64_t>((cs.mid_n + 3) / 4, ctx->tuning.blocks_per_bucket);
             VT_LAUNCH(ctx, "local_move_mid", kern, (unsigned)((nb + kXcds - 1) / kXcds * kXcds), 256, 0, a,
                       (const int32_t*)(lists.p + cs.mid_off), cs.mid_n);
@@ -2416,6 +2469,7 @@ vt_tuning default_tuning()
     t.blocks_per_bucket = 16384;
     t.wide_table_max_degree[0] = 255;  // tables at most half full
     t.wide_table_max_degree[1] = 511;
+    t.vertex_records = 0;
     return t;
 }
 
@@ -2486,6 +2540,8 @@ static void check_tuning(vt_ctx* ctx, const vt_tuning* t)
     if (w0 < -1 || w0 > 3 * 512 / 4 - 1 || w1 < -1 || w1 > 3 * 1024 / 4 - 1 || w0 > w1)
         fail(ctx, VT_ERR_ARG, "vt_tuning: wide_table_max_degree = {%d, %d}: need -1 <= [0] <= [1], [0] <= 383, [1] <= 767",
              w0, w1);
+    if (t->vertex_records != 0 && t->vertex_records != 1)
+        fail(ctx, VT_ERR_ARG, "vt_tuning: vertex_records = %d, need 0 or 1", t->vertex_records);
 }
 
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out)
@@ -2504,9 +2560,10 @@ int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t)
     });
 }
 
-// Coordinate descent over the bucket bounds (one chunk or up to two per row at each group
-// size) and the per-bucket block cap, each candidate timed with HIP events over `reps`
-// sweeps of the same state; a candidate must win by 1% to replace the incumbent.
+// Coordinate descent over the community-record layout, the bucket bounds (one chunk or up to
+// two per row at each group size), the per-bucket block cap and the wide-row table sizes,
+// each candidate timed with HIP events over `reps` sweeps of the same state; a candidate
+// must win by 1% to replace the incumbent.
 int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning* out)
 {
     if (!ctx || !g || prep < 0 || reps < 1) return VT_ERR_ARG;
@@ -2539,6 +2596,18 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
             };
             vt_tuning best = start;
             double tb = time_it(best);
+            // community records against separate arrays: first, since it changes which
+            // bucket bounds and caps are best, and again at the end
+            auto try_records = [&]() {
+                vt_tuning t = best;
+                t.vertex_records = 1 - best.vertex_records;
+                const double tt = time_it(t);
+                if (tt < 0.99 * tb) {
+                    best = t;
+                    tb = tt;
+                }
+            };
+            try_records();
             for (int b = 0; b < 5; b++) {
                 const int G = 4 << b;
                 std::vector<int> cands;
@@ -2584,6 +2653,7 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
                     }
                 }
             }
+            try_records();
             ctx->profiling = prof;
             ctx->tuning = best;
         } catch (...) {
