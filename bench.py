@@ -188,6 +188,9 @@ def iteration_timing(st, V, E, steps=3):
             "bytes_formula": "24E + 88V + 64 per mover over the iteration's wall time"}
 
 
+LAST_N = 5  # timed sweeps in a profiling run (bench.py --steps 5)
+
+
 def summarize_profile(d):
     """Condenses rocprofv3 CSV output under d (kernel stats, kernel trace, counter
     collection) into a short text summary: per kernel, dispatches and mean duration, and
@@ -211,15 +214,42 @@ def summarize_profile(d):
                     if i < 25:
                         lines.append(f"{short(row['Name'])} | {row['Calls']} | {float(row['AverageNs']) / 1e6:.4f} | "
                                      f"{float(row['TotalDurationNs']) / 1e6:.3f} | {float(row['Percentage']):.2f}")
+        elif base.endswith("kernel_trace.csv"):
+            # the local-move launches one by one, in dispatch order: the timed sweeps of a
+            # bench run are the last ones (profile with --no-iter --no-ttq)
+            per = collections.defaultdict(list)
+            with open(f) as fh:
+                rows = [r for r in csv.DictReader(fh) if "local_move" in r.get("Kernel_Name", "")]
+            rows.sort(key=lambda r: int(r.get("Start_Timestamp", 0) or 0))
+            for r in rows:
+                per[short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+            if per:
+                lines.append(f"## {base}: local-move launches: kernel | dispatches | last {LAST_N} ms each | "
+                             f"mean of the last {LAST_N}")
+                for k, v in per.items():
+                    tail = v[-LAST_N:]
+                    lines.append(f"{k} | {len(v)} | {' '.join(f'{x:.4f}' for x in tail)} | "
+                                 f"{sum(tail) / len(tail):.4f}")
         elif base.endswith("counter_collection.csv"):
             agg = collections.defaultdict(lambda: collections.defaultdict(list))
             with open(f) as fh:
-                for row in csv.DictReader(fh):
-                    agg[short(row.get("Kernel_Name", "?"))][row.get("Counter_Name", "?")].append(
-                        float(row.get("Counter_Value", 0)))
+                rows = list(csv.DictReader(fh))
+            if rows and "Dispatch_Id" in rows[0]:
+                rows.sort(key=lambda r: int(r["Dispatch_Id"] or 0))
+            for row in rows:
+                agg[short(row.get("Kernel_Name", "?"))][row.get("Counter_Name", "?")].append(
+                    float(row.get("Counter_Value", 0)))
             lines.append(f"## {base}: kernel | counter | dispatches | mean per dispatch")
             for k, cs in agg.items():
                 for c, v in cs.items():
+                    lines.append(f"{k} | {c} | {len(v)} | {sum(v) / len(v):.6g}")
+            # the local-move launches again, over their last dispatches only (the timed
+            # sweeps); listed after the means, so a reader of these lines takes these
+            last = [(k, c, v[-LAST_N:]) for k, cs in agg.items() if "local_move" in k for c, v in cs.items()]
+            if last:
+                lines.append(f"## {base}, local-move launches, last {LAST_N} dispatches: kernel | counter | "
+                             f"dispatches | mean per dispatch")
+                for k, c, v in last:
                     lines.append(f"{k} | {c} | {len(v)} | {sum(v) / len(v):.6g}")
     return "\n".join(lines) + "\n"
 
@@ -568,6 +598,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttq", action="store_true", help="skip the time-to-final-Q run")
+    ap.add_argument("--no-iter", action="store_true", help="skip the whole-iteration timing (for profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batched-form run (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=256,
@@ -636,9 +667,11 @@ def main():
     kavg, kcount, ktot = ctx.kernel_stats("local_move")
     ctx.set_profiling(False)
     state = st.state()
-    per_iter = iteration_timing(st, V, E)
-    log(f"whole iterations: {per_iter['ms_per_iteration']:.3f} ms each "
-        f"({100 * per_iter['hbm_frac']:.1f}% of HBM peak in algorithmic bytes)")
+    per_iter = None
+    if not args.no_iter:
+        per_iter = iteration_timing(st, V, E)
+        log(f"whole iterations: {per_iter['ms_per_iteration']:.3f} ms each "
+            f"({100 * per_iter['hbm_frac']:.1f}% of HBM peak in algorithmic bytes)")
     st.end(want_cvect=False)
 
     # the dominant kernel's own share: the rows of bucket 2 under the tuning in force

@@ -76,3 +76,23 @@ def test_degree_buckets_follow_the_bounds():
     # an empty bucket (-1) passes its rows to the next one
     b = bench.degree_buckets(deg, [7, 15, -1, 63, 191, 1023])
     assert b.tolist() == [0, 0, 0, 1, 1, 3, 3, 3, 3, 4, 5, 6]
+
+
+def test_summarize_profile_lists_last_local_move_dispatches(tmp_path):
+    """Kernel traces: the local-move launches one by one in start order, with the mean of the
+    last five (a profiling run's timed sweeps); counters: the same kernels over their last
+    five dispatches, listed after the all-dispatch means."""
+    d = str(tmp_path / "p")
+    K3 = "void vt::k_local_move<false, 2, false>(vt::MoveArgs, int const*, vt::SweepPlan)"
+    rows = [[K3, 1000 * i, 1000 * i + ns] for i, ns in enumerate([9_000_000, 8_000_000] + [2_000_000] * 5)]
+    rows.insert(3, [K2, 10, 20])
+    _write(os.path.join(d, "t", "kt_kernel_trace.csv"), ["Kernel_Name", "Start_Timestamp", "End_Timestamp"],
+           list(reversed(rows)))
+    cc = [[K3, "FETCH_SIZE", v, i] for i, v in enumerate([90.0, 80.0, 10.0, 10.0, 10.0, 10.0, 10.0])]
+    _write(os.path.join(d, "c", "f_counter_collection.csv"), ["Kernel_Name", "Counter_Name", "Counter_Value",
+                                                              "Dispatch_Id"], cc)
+    text = bench.summarize_profile(d)
+    assert "void vt::k_local_move<false, 2, false> | 7 | 2.0000 2.0000 2.0000 2.0000 2.0000 | 2.0000" in text
+    lines = text.splitlines()
+    mean_all = lines.index("void vt::k_local_move<false, 2, false> | FETCH_SIZE | 7 | 31.4286")
+    assert lines.index("void vt::k_local_move<false, 2, false> | FETCH_SIZE | 5 | 10") > mean_all

@@ -56,22 +56,24 @@ __device__ __forceinline__ int64_t block_inclusive_scan(int64_t x, int64_t* s_wa
     return x + before;
 }
 
-// Block partial sums of in[0, n).
+// Chunk partial sums of in[0, n), chunks of 2048 values dealt to the blocks in turn.
 __global__ void __launch_bounds__(kScanBlock) k_dscan_reduce(const int32_t* __restrict__ in, const int64_t* __restrict__ d_n,
                                                            int64_t* __restrict__ part)
 {
     __shared__ int64_t s_wave[kScanBlock / 64];
     const int64_t n = *d_n;
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
-    if (base >= n) return;
-    int64_t x = 0;
-    for (int k = 0; k < kScanItems; k++) {
-        const int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
-        if (i < n) x += in[i];
+    const int64_t nb = ceil_div(n, kScanChunk);
+    for (int64_t c = blockIdx.x; c < nb; c += gridDim.x) {
+        const int64_t base = c * kScanChunk;
+        int64_t x = 0;
+        for (int k = 0; k < kScanItems; k++) {
+            const int64_t i = base + (int64_t)k * kScanBlock + threadIdx.x;
+            if (i < n) x += in[i];
+        }
+        int64_t total = 0;
+        block_inclusive_scan(x, s_wave, &total);
+        if (threadIdx.x == 0) part[c] = total;
     }
-    int64_t total = 0;
-    block_inclusive_scan(x, s_wave, &total);
-    if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
 // Exclusive scan of the block partials in place (one block), and the grand total.
@@ -104,27 +106,28 @@ __global__ void __launch_bounds__(kPartBlock) k_dscan_parts(const int64_t* __res
     if (threadIdx.x == 0 && d_total) *d_total = carry;
 }
 
-// Exclusive scan of each block's values, offset by the block's scanned partial.
+// Exclusive scan of each chunk's values, offset by the chunk's scanned partial.
 __global__ void __launch_bounds__(kScanBlock) k_dscan_down(const int32_t* __restrict__ in, const int64_t* __restrict__ d_n,
                                                          const int64_t* __restrict__ part, int32_t* __restrict__ out)
 {
     __shared__ int64_t s_wave[kScanBlock / 64];
     const int64_t n = *d_n;
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
-    if (base >= n) return;
-    // thread t scans the contiguous run [t*8, t*8+8) of the chunk
-    const int64_t lo = base + (int64_t)threadIdx.x * kScanItems;
-    int32_t v[kScanItems];
-    int64_t x = 0;
-    for (int k = 0; k < kScanItems; k++) {
-        v[k] = (lo + k < n) ? in[lo + k] : 0;
-        x += v[k];
-    }
-    int64_t total = 0;
-    int64_t run = block_inclusive_scan(x, s_wave, &total) - x + part[blockIdx.x];
-    for (int k = 0; k < kScanItems; k++) {
-        if (lo + k < n) out[lo + k] = (int32_t)run;
-        run += v[k];
+    const int64_t nb = ceil_div(n, kScanChunk);
+    for (int64_t c = blockIdx.x; c < nb; c += gridDim.x) {
+        // thread t scans the contiguous run [t*8, t*8+8) of the chunk
+        const int64_t lo = c * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+        int32_t v[kScanItems];
+        int64_t x = 0;
+        for (int k = 0; k < kScanItems; k++) {
+            v[k] = (lo + k < n) ? in[lo + k] : 0;
+            x += v[k];
+        }
+        int64_t total = 0;
+        int64_t run = block_inclusive_scan(x, s_wave, &total) - x + part[c];
+        for (int k = 0; k < kScanItems; k++) {
+            if (lo + k < n) out[lo + k] = (int32_t)run;
+            run += v[k];
+        }
     }
 }
 
@@ -150,16 +153,17 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_hist(const uint32_t* __res
 {
     __shared__ int32_t s_hist[256];
     const int64_t n = sizes[0], nt = sizes[1];
-    const int64_t b = blockIdx.x;
-    if (b >= nt) return;
-    s_hist[threadIdx.x] = 0;
-    __syncthreads();
-    for (int r = 0; r < kSortRounds; r++) {
-        const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
-        if (i < n) atomicAdd(&s_hist[digit_of(keys[i], shift, dmask)], 1);
+    for (int64_t b = blockIdx.x; b < nt; b += gridDim.x) {
+        s_hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int r = 0; r < kSortRounds; r++) {
+            const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
+            if (i < n) atomicAdd(&s_hist[digit_of(keys[i], shift, dmask)], 1);
+        }
+        __syncthreads();
+        hist[(int64_t)threadIdx.x * nt + b] = s_hist[threadIdx.x];
+        __syncthreads();
     }
-    __syncthreads();
-    hist[(int64_t)threadIdx.x * nt + b] = s_hist[threadIdx.x];
 }
 
 // Places the pairs of one tile in two steps.  First every pair gets its place in the tile
@@ -183,11 +187,9 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __
     __shared__ int64_t s_wave[kWaves];
     __shared__ uint32_t s_key[kTile], s_val[kTile];
     const int64_t n = sizes[0], nt = sizes[1];
-    const int64_t b = blockIdx.x;
-    if (b >= nt) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1;
-    {
+    for (int64_t b = blockIdx.x; b < nt; b += gridDim.x) {
         // the tile's digit counts, scanned: first staged place of each digit
         const int32_t cnt = hist[(int64_t)threadIdx.x * nt + b];
         int64_t total = 0;
@@ -195,60 +197,66 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __
         s_first[threadIdx.x] = first;
         s_out[threadIdx.x] = offs[(int64_t)threadIdx.x * nt + b] - first;
         s_run[threadIdx.x] = 0;
-    }
-    const int64_t base = b * kTile;
-    const int tile_n = (int)(n - base < kTile ? n - base : kTile);
-    for (int r = 0; r < kSortRounds; r++) {
-        for (int k = 0; k < kWaves; k++) s_wcnt[k][threadIdx.x] = 0;
-        __syncthreads();
-        const int li = r * kSortBlock + threadIdx.x;
-        const bool valid = li < tile_n;
-        uint32_t key = 0, val = 0;
-        if (valid) {
-            key = keys[base + li];
-            val = vals[base + li];
+        const int64_t base = b * kTile;
+        const int tile_n = (int)(n - base < kTile ? n - base : kTile);
+        for (int r = 0; r < kSortRounds; r++) {
+            for (int k = 0; k < kWaves; k++) s_wcnt[k][threadIdx.x] = 0;
+            __syncthreads();
+            const int li = r * kSortBlock + threadIdx.x;
+            const bool valid = li < tile_n;
+            uint32_t key = 0, val = 0;
+            if (valid) {
+                key = keys[base + li];
+                val = vals[base + li];
+            }
+            const uint32_t d = digit_of(key, shift, dmask);
+            // lanes of this wave holding the same digit
+            unsigned long long peers = __ballot(valid);
+            for (int bit = 0; bit < 8; bit++) {
+                const bool set = (d >> bit) & 1u;
+                const unsigned long long m = __ballot(set);
+                peers &= set ? m : ~m;
+            }
+            const int rank = __popcll(peers & below);
+            if (valid && rank == 0) s_wcnt[w][d] = __popcll(peers);
+            __syncthreads();
+            if (valid) {
+                int place = s_first[d] + s_run[d] + rank;
+                for (int k = 0; k < w; k++) place += s_wcnt[k][d];
+                s_key[place] = key;
+                s_val[place] = val;
+            }
+            __syncthreads();
+            int add = 0;
+            for (int k = 0; k < kWaves; k++) add += s_wcnt[k][threadIdx.x];
+            s_run[threadIdx.x] += add;
         }
-        const uint32_t d = digit_of(key, shift, dmask);
-        // lanes of this wave holding the same digit
-        unsigned long long peers = __ballot(valid);
-        for (int bit = 0; bit < 8; bit++) {
-            const bool set = (d >> bit) & 1u;
-            const unsigned long long m = __ballot(set);
-            peers &= set ? m : ~m;
-        }
-        const int rank = __popcll(peers & below);
-        if (valid && rank == 0) s_wcnt[w][d] = __popcll(peers);
         __syncthreads();
-        if (valid) {
-            int place = s_first[d] + s_run[d] + rank;
-            for (int k = 0; k < w; k++) place += s_wcnt[k][d];
-            s_key[place] = key;
-            s_val[place] = val;
+        for (int p = threadIdx.x; p < tile_n; p += kSortBlock) {
+            const uint32_t key = s_key[p];
+            const int64_t pos = (int64_t)s_out[digit_of(key, shift, dmask)] + p;
+            keys_out[pos] = key;
+            vals_out[pos] = s_val[p];
         }
-        __syncthreads();
-        int add = 0;
-        for (int k = 0; k < kWaves; k++) add += s_wcnt[k][threadIdx.x];
-        s_run[threadIdx.x] += add;
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < tile_n; p += kSortBlock) {
-        const uint32_t key = s_key[p];
-        const int64_t pos = (int64_t)s_out[digit_of(key, shift, dmask)] + p;
-        keys_out[pos] = key;
-        vals_out[pos] = s_val[p];
+        __syncthreads();  // the staging arrays are refilled for the next tile
     }
 }
 
 }  // namespace
 
+// Blocks per launch: enough to fill the device, few enough that a small count leaves few
+// blocks with nothing to do (the count is not known on the host).
+constexpr int64_t kMaxGrid = 2048;
+
 void dev_scan_i32(vt_ctx* ctx, const int32_t* in, int32_t* out, const int64_t* d_n, int64_t n_max, int64_t* d_total,
                   const char* part_key)
 {
     const int64_t nb_max = std::max<int64_t>(1, (n_max + kScanChunk - 1) / kScanChunk);
+    const unsigned grid = (unsigned)std::min(nb_max, kMaxGrid);
     int64_t* part = (int64_t*)scratch(ctx, part_key, 8 * (size_t)nb_max);
-    VT_LAUNCH(ctx, "dscan_reduce", k_dscan_reduce, (unsigned)nb_max, kScanBlock, 0, in, d_n, part);
+    VT_LAUNCH(ctx, "dscan_reduce", k_dscan_reduce, grid, kScanBlock, 0, in, d_n, part);
     VT_LAUNCH(ctx, "dscan_parts", k_dscan_parts, 1, kPartBlock, 0, d_n, part, d_total);
-    VT_LAUNCH(ctx, "dscan_down", k_dscan_down, (unsigned)nb_max, kScanBlock, 0, in, d_n, (const int64_t*)part, out);
+    VT_LAUNCH(ctx, "dscan_down", k_dscan_down, grid, kScanBlock, 0, in, d_n, (const int64_t*)part, out);
 }
 
 bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
@@ -257,6 +265,7 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
     if (n_max >= (int64_t(1) << 31))
         fail(ctx, VT_ERR_LIMIT, "device sort of up to %lld pairs: positions must fit 31 bits", (long long)n_max);
     const int64_t nt_max = std::max<int64_t>(1, (n_max + kTile - 1) / kTile);
+    const unsigned grid = (unsigned)std::min(nt_max, kMaxGrid);
     int64_t* sizes = (int64_t*)scratch(ctx, "dsort_sizes", 8 * 4);
     int32_t* hist = (int32_t*)scratch(ctx, "dsort_hist", 4 * 256 * (size_t)nt_max);
     int32_t* offs = (int32_t*)scratch(ctx, "dsort_offs", 4 * 256 * (size_t)nt_max);
@@ -268,10 +277,10 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
         uint32_t* vi = alt ? vals_alt : vals;
         uint32_t* ko = alt ? keys : keys_alt;
         uint32_t* vo = alt ? vals : vals_alt;
-        VT_LAUNCH(ctx, "dsort_hist", k_dsort_hist, (unsigned)nt_max, kSortBlock, 0, (const uint32_t*)ki,
+        VT_LAUNCH(ctx, "dsort_hist", k_dsort_hist, grid, kSortBlock, 0, (const uint32_t*)ki,
                   (const int64_t*)sizes, shift, dmask, hist);
         dev_scan_i32(ctx, hist, offs, sizes + 2, 256 * nt_max, nullptr, "dsort_part");
-        VT_LAUNCH(ctx, "dsort_scatter", k_dsort_scatter, (unsigned)nt_max, kSortBlock, 0, (const uint32_t*)ki,
+        VT_LAUNCH(ctx, "dsort_scatter", k_dsort_scatter, grid, kSortBlock, 0, (const uint32_t*)ki,
                   (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, (const int32_t*)hist,
                   (const int32_t*)offs, ko, vo);
         alt = !alt;
