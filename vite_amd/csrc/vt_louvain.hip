@@ -2561,7 +2561,8 @@ int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t)
 }
 
 // Coordinate descent over the community-record layout, the bucket bounds (one chunk or up to
-// two per row at each group size), the per-bucket block cap and the wide-row table sizes,
+// two per row at each group size), the per-bucket block cap (then the bounds again when the
+// cap changed) and the wide-row table sizes,
 // each candidate timed with HIP events over `reps` sweeps of the same state; a candidate
 // must win by 1% to replace the incumbent.
 int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning* out)
@@ -2608,21 +2609,25 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
                 }
             };
             try_records();
-            for (int b = 0; b < 5; b++) {
-                const int G = 4 << b;
-                std::vector<int> cands;
-                for (int h = 2; h <= 2 * kNch[b]; h++) cands.push_back(h * G / 2 - 1);  // 1, 1.5, 2, ... chunks
-                for (int cand : cands) {
-                    if (cand == best.bucket_max_degree[b]) continue;
-                    vt_tuning t = best;
-                    t.bucket_max_degree[b] = cand;
-                    const double tt = time_it(t);
-                    if (tt < 0.99 * tb) {
-                        best = t;
-                        tb = tt;
+            auto try_bounds = [&]() {
+                for (int b = 0; b < 5; b++) {
+                    const int G = 4 << b;
+                    std::vector<int> cands;
+                    for (int h = 2; h <= 2 * kNch[b]; h++) cands.push_back(h * G / 2 - 1);  // 1, 1.5, 2, ... chunks
+                    for (int cand : cands) {
+                        if (cand == best.bucket_max_degree[b]) continue;
+                        vt_tuning t = best;
+                        t.bucket_max_degree[b] = cand;
+                        const double tt = time_it(t);
+                        if (tt < 0.99 * tb) {
+                            best = t;
+                            tb = tt;
+                        }
                     }
                 }
-            }
+            };
+            try_bounds();
+            const vt_tuning after_bounds = best;
             for (int cap : {2048, 4096, 8192, 16384, 65536}) {
                 if (cap == best.blocks_per_bucket) continue;
                 vt_tuning t = best;
@@ -2633,6 +2638,8 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
                     tb = tt;
                 }
             }
+            // a new cap moves the best bounds: one more pass over them
+            if (best.blocks_per_bucket != after_bounds.blocks_per_bucket) try_bounds();
             // wide-row table sizes: fuller small tables (more waves per CU) against emptier
             // large ones (shorter probes), when the graph has wide rows at all
             const ColorSweep& cs0 = st->ph.sweeps[0];

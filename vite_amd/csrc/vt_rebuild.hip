@@ -201,26 +201,30 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
         int nd = 0;  // distinct external keys so far (the table never holds more than kCfList)
         double self = -0.0;  // the internal entries' fold, the same in every lane
         bool any_self = false;
-        DEdge e{};  // this lane's entry of the current chunk
+        // This lane's entry of the current chunk and its coarse column, and the next chunk's
+        // entry: a chunk's column reads and the entries of the chunk after it are in flight
+        // while the chunk before is folded, so a chunk waits on no load of its own.
+        DEdge e{}, en{};
+        uint32_t c2 = 0;
         if (mr.s0 < mr.s1) {
             const int64_t k = mr.csr_index(mr.s0 + lane, moff, mrow);
             if (mr.s0 + lane < mr.s1) e = edges[k];
+            if (mr.s0 + 64 < mr.s1) {
+                const int64_t kn = mr.csr_index(mr.s0 + 64 + lane, moff, mrow);
+                if (mr.s0 + 64 + lane < mr.s1) en = edges[kn];
+            }
+            if (mr.s0 + lane < mr.s1) c2 = newid[e.tail];
         }
         for (int64_t base = mr.s0; base < mr.s1 && nd <= kCfMaxKeys; base += 64) {
             const int64_t p = base + lane;
             const bool valid = p < mr.s1;
-            uint32_t c2 = 0;
-            double wt = -0.0;
-            if (valid) {
-                c2 = newid[e.tail];
-                wt = e.weight;
-            }
-            // the next chunk's entries are loaded while this one is folded (issued after the
-            // gather above, so waiting for the gather does not wait for them)
-            DEdge en{};
-            if (base + 64 < mr.s1) {
-                const int64_t kn = mr.csr_index(p + 64, moff, mrow);
-                if (p + 64 < mr.s1) en = edges[kn];
+            const double wt = valid ? e.weight : -0.0;
+            uint32_t c2n = 0;
+            if (p + 64 < mr.s1) c2n = newid[en.tail];
+            DEdge enn{};
+            if (base + 128 < mr.s1) {
+                const int64_t kn = mr.csr_index(p + 128, moff, mrow);
+                if (p + 128 < mr.s1) enn = edges[kn];
             }
             // internal entries (about half after a phase) form one long run per chunk: add
             // them in lane order with the lane read as a scalar, without the table
@@ -261,6 +265,8 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
             }
             wave_lds_sync();
             e = en;
+            en = enn;
+            c2 = c2n;
         }
         if (nd <= kCfMaxKeys) {
             // each external key's column is its rank among the (distinct) keys, one further
