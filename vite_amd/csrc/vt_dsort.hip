@@ -7,7 +7,7 @@
 // at once.  They record no memset, allocate nothing once their scratch exists, and make no
 // host read, so a sequence of them can be captured and replayed.
 //
-// Sort: least-significant-digit radix sort, 8-bit digits, tiles of 2048 pairs.  Per pass a
+// Sort: least-significant-digit radix sort, 8- or 9-bit digits, tiles of 2048 pairs.  Per pass a
 // histogram kernel counts each tile's digits (digit-major table: digit d of tile b at
 // d * ntiles + b), the table is scanned, and a scatter kernel places each pair at its digit's
 // offset for its tile plus its rank among the tile's pairs of that digit in input order.
@@ -32,6 +32,9 @@ constexpr int kSortBlock = 256;
 constexpr int kSortRounds = 8;
 constexpr int64_t kTile = (int64_t)kSortBlock * kSortRounds;  // 2048 pairs per tile
 constexpr int kWaves = kSortBlock / 64;
+constexpr int kMaxDigitBits = 9;                // digits of 8 or 9 bits
+constexpr int kMaxBins = 1 << kMaxDigitBits;    // bins of a 9-bit digit
+constexpr int kMaxBinsPerThread = kMaxBins / kSortBlock;
 
 __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -131,37 +134,37 @@ __global__ void __launch_bounds__(kScanBlock) k_dscan_down(const int32_t* __rest
     }
 }
 
-// Sizes of one sort: pairs, tiles and histogram entries (256 per tile).
-__global__ void k_dsort_sizes(const int64_t* __restrict__ d_count, int64_t mult, int64_t* __restrict__ sizes)
+// Sizes of one sort: pairs, tiles and histogram entries (nbins per tile).
+__global__ void k_dsort_sizes(const int64_t* __restrict__ d_count, int64_t mult, int nbins, int64_t* __restrict__ sizes)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const int64_t n = *d_count * mult;
         const int64_t nt = ceil_div(n, kTile);
         sizes[0] = n;
         sizes[1] = nt;
-        sizes[2] = 256 * nt;
+        sizes[2] = nbins * nt;
     }
 }
 
-// The pass's digit: key bits [shift, shift + 8), cut at the sort's last bit (dmask).
+// The pass's digit: key bits [shift, shift + digit bits), cut at the sort's last bit (dmask).
 __device__ __forceinline__ uint32_t digit_of(uint32_t key, int shift, uint32_t dmask) { return (key >> shift) & dmask; }
 
-// Digit counts of each tile, digit-major.
+// Digit counts of each tile, digit-major (nbins = 256 or 512 bins).
 __global__ void __launch_bounds__(kSortBlock) k_dsort_hist(const uint32_t* __restrict__ keys,
                                                          const int64_t* __restrict__ sizes, int shift, uint32_t dmask,
-                                                         int32_t* __restrict__ hist)
+                                                         int nbins, int32_t* __restrict__ hist)
 {
-    __shared__ int32_t s_hist[256];
+    __shared__ int32_t s_hist[kMaxBins];
     const int64_t n = sizes[0], nt = sizes[1];
     for (int64_t b = blockIdx.x; b < nt; b += gridDim.x) {
-        s_hist[threadIdx.x] = 0;
+        for (int d = threadIdx.x; d < nbins; d += kSortBlock) s_hist[d] = 0;
         __syncthreads();
         for (int r = 0; r < kSortRounds; r++) {
             const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
             if (i < n) atomicAdd(&s_hist[digit_of(keys[i], shift, dmask)], 1);
         }
         __syncthreads();
-        hist[(int64_t)threadIdx.x * nt + b] = s_hist[threadIdx.x];
+        for (int d = threadIdx.x; d < nbins; d += kSortBlock) hist[(int64_t)d * nt + b] = s_hist[d];
         __syncthreads();
     }
 }
@@ -175,32 +178,44 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_hist(const uint32_t* __res
 __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ vals,
                                                             const int64_t* __restrict__ sizes, int shift,
-                                                            uint32_t dmask, const int32_t* __restrict__ hist,
+                                                            uint32_t dmask, int dbits, const int32_t* __restrict__ hist,
                                                             const int32_t* __restrict__ offs,
                                                             uint32_t* __restrict__ keys_out,
                                                             uint32_t* __restrict__ vals_out)
 {
-    __shared__ int32_t s_run[256];              // digit's pairs placed by earlier rounds
-    __shared__ int32_t s_wcnt[kWaves][256];     // this round's pairs per wave and digit
-    __shared__ int32_t s_first[256];            // first staged place of each digit
-    __shared__ int32_t s_out[256];              // global place of each digit's first pair, minus s_first
+    __shared__ int32_t s_run[kMaxBins];            // digit's pairs placed by earlier rounds
+    __shared__ int32_t s_wcnt[kWaves][kMaxBins];   // this round's pairs per wave and digit
+    __shared__ int32_t s_first[kMaxBins];          // first staged place of each digit
+    __shared__ int32_t s_out[kMaxBins];            // global place of each digit's first pair, minus s_first
     __shared__ int64_t s_wave[kWaves];
     __shared__ uint32_t s_key[kTile], s_val[kTile];
     const int64_t n = sizes[0], nt = sizes[1];
+    const int nbins = 1 << dbits;
+    const int bpt = nbins / kSortBlock;  // bins per thread, consecutive: 1 or 2
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1;
     for (int64_t b = blockIdx.x; b < nt; b += gridDim.x) {
         // the tile's digit counts, scanned: first staged place of each digit
-        const int32_t cnt = hist[(int64_t)threadIdx.x * nt + b];
+        int32_t cnt[kMaxBinsPerThread];
+        int32_t mine = 0;
+        for (int j = 0; j < bpt; j++) {
+            cnt[j] = hist[(int64_t)(threadIdx.x * bpt + j) * nt + b];
+            mine += cnt[j];
+        }
         int64_t total = 0;
-        const int32_t first = (int32_t)(block_inclusive_scan(cnt, s_wave, &total) - cnt);
-        s_first[threadIdx.x] = first;
-        s_out[threadIdx.x] = offs[(int64_t)threadIdx.x * nt + b] - first;
-        s_run[threadIdx.x] = 0;
+        int32_t first = (int32_t)(block_inclusive_scan(mine, s_wave, &total) - mine);
+        for (int j = 0; j < bpt; j++) {
+            const int d = threadIdx.x * bpt + j;
+            s_first[d] = first;
+            s_out[d] = offs[(int64_t)d * nt + b] - first;
+            s_run[d] = 0;
+            first += cnt[j];
+        }
         const int64_t base = b * kTile;
         const int tile_n = (int)(n - base < kTile ? n - base : kTile);
         for (int r = 0; r < kSortRounds; r++) {
-            for (int k = 0; k < kWaves; k++) s_wcnt[k][threadIdx.x] = 0;
+            for (int k = 0; k < kWaves; k++)
+                for (int d = threadIdx.x; d < nbins; d += kSortBlock) s_wcnt[k][d] = 0;
             __syncthreads();
             const int li = r * kSortBlock + threadIdx.x;
             const bool valid = li < tile_n;
@@ -212,7 +227,7 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __
             const uint32_t d = digit_of(key, shift, dmask);
             // lanes of this wave holding the same digit
             unsigned long long peers = __ballot(valid);
-            for (int bit = 0; bit < 8; bit++) {
+            for (int bit = 0; bit < dbits; bit++) {
                 const bool set = (d >> bit) & 1u;
                 const unsigned long long m = __ballot(set);
                 peers &= set ? m : ~m;
@@ -227,9 +242,11 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __
                 s_val[place] = val;
             }
             __syncthreads();
-            int add = 0;
-            for (int k = 0; k < kWaves; k++) add += s_wcnt[k][threadIdx.x];
-            s_run[threadIdx.x] += add;
+            for (int dd = threadIdx.x; dd < nbins; dd += kSortBlock) {
+                int add = 0;
+                for (int k = 0; k < kWaves; k++) add += s_wcnt[k][dd];
+                s_run[dd] += add;
+            }
         }
         __syncthreads();
         for (int p = threadIdx.x; p < tile_n; p += kSortBlock) {
@@ -266,22 +283,26 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
         fail(ctx, VT_ERR_LIMIT, "device sort of up to %lld pairs: positions must fit 31 bits", (long long)n_max);
     const int64_t nt_max = std::max<int64_t>(1, (n_max + kTile - 1) / kTile);
     const unsigned grid = (unsigned)std::min(nt_max, kMaxGrid);
+    // 8-bit digits up to 24 key bits (three passes at most), else 9-bit digits: 25-27 bits
+    // (the move events of up to 2^27 vertices) in three passes instead of four
+    const int dbits = bits > 24 ? kMaxDigitBits : 8;
+    const int nbins = 1 << dbits;
     int64_t* sizes = (int64_t*)scratch(ctx, "dsort_sizes", 8 * 4);
-    int32_t* hist = (int32_t*)scratch(ctx, "dsort_hist", 4 * 256 * (size_t)nt_max);
-    int32_t* offs = (int32_t*)scratch(ctx, "dsort_offs", 4 * 256 * (size_t)nt_max);
-    VT_LAUNCH(ctx, "dsort_sizes", k_dsort_sizes, 1, 64, 0, d_count, mult, sizes);
+    int32_t* hist = (int32_t*)scratch(ctx, "dsort_hist", 4 * (size_t)nbins * (size_t)nt_max);
+    int32_t* offs = (int32_t*)scratch(ctx, "dsort_offs", 4 * (size_t)nbins * (size_t)nt_max);
+    VT_LAUNCH(ctx, "dsort_sizes", k_dsort_sizes, 1, 64, 0, d_count, mult, nbins, sizes);
     bool alt = false;
-    for (int shift = 0; shift < bits; shift += 8) {
-        const uint32_t dmask = bits - shift >= 8 ? 255u : (1u << (bits - shift)) - 1u;
+    for (int shift = 0; shift < bits; shift += dbits) {
+        const uint32_t dmask = bits - shift >= dbits ? (uint32_t)nbins - 1u : (1u << (bits - shift)) - 1u;
         uint32_t* ki = alt ? keys_alt : keys;
         uint32_t* vi = alt ? vals_alt : vals;
         uint32_t* ko = alt ? keys : keys_alt;
         uint32_t* vo = alt ? vals : vals_alt;
         VT_LAUNCH(ctx, "dsort_hist", k_dsort_hist, grid, kSortBlock, 0, (const uint32_t*)ki,
-                  (const int64_t*)sizes, shift, dmask, hist);
-        dev_scan_i32(ctx, hist, offs, sizes + 2, 256 * nt_max, nullptr, "dsort_part");
+                  (const int64_t*)sizes, shift, dmask, nbins, hist);
+        dev_scan_i32(ctx, hist, offs, sizes + 2, (int64_t)nbins * nt_max, nullptr, "dsort_part");
         VT_LAUNCH(ctx, "dsort_scatter", k_dsort_scatter, grid, kSortBlock, 0, (const uint32_t*)ki,
-                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, (const int32_t*)hist,
+                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, dbits, (const int32_t*)hist,
                   (const int32_t*)offs, ko, vo);
         alt = !alt;
     }
