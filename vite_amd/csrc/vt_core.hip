@@ -299,10 +299,12 @@ __global__ void __launch_bounds__(kSumBlock) k_seg_sum2_partial(const double* __
                                                                 const int32_t* __restrict__ bgraph,
                                                                 const int32_t* __restrict__ bsub,
                                                                 const int64_t* __restrict__ chunk,
+                                                                const unsigned char* __restrict__ gact,
                                                                 double2* __restrict__ part)
 {
     __shared__ double sa[kSumBlock], sb[kSumBlock];
     const int32_t g = bgraph[blockIdx.x];
+    if (gact && !gact[g]) return;  // a graph that has left the loop: its sums are not read
     const int64_t base = voff[g], n = voff[g + 1] - base, ch = chunk[g];
     const int64_t lo = (int64_t)bsub[blockIdx.x] * ch;
     const int64_t hi = (lo + ch < n) ? lo + ch : n;
@@ -326,10 +328,12 @@ __global__ void __launch_bounds__(kSumBlock) k_seg_sum2_partial(const double* __
 
 __global__ void __launch_bounds__(kSumBlock) k_seg_sum2_final(const double2* __restrict__ part,
                                                               const int64_t* __restrict__ pbase,
+                                                              const unsigned char* __restrict__ gact,
                                                               double* __restrict__ out)
 {
     __shared__ double sa[kSumBlock], sb[kSumBlock];
     const int32_t g = blockIdx.x;
+    if (gact && !gact[g]) return;
     const double2* p = part + pbase[g];
     const int np = (int)(pbase[g + 1] - pbase[g]);
     double xa = 0.0, xb = 0.0;
@@ -349,9 +353,11 @@ __global__ void __launch_bounds__(kSumBlock) k_seg_sum2_final(const double2* __r
 
 // Left folds per graph, as k_seq_sum2 forms them for one graph.
 __global__ void k_seg_seq_sum2(const double* __restrict__ a, const double* __restrict__ b,
-                               const int64_t* __restrict__ voff, int32_t ng, double* __restrict__ out)
+                               const int64_t* __restrict__ voff, int32_t ng, const unsigned char* __restrict__ gact,
+                               double* __restrict__ out)
 {
     for (int32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+        if (gact && !gact[g]) continue;
         double xa = 0.0, xb = 0.0;
         for (int64_t i = voff[g]; i < voff[g + 1]; i++) {
             xa = xa + a[i];
@@ -395,21 +401,23 @@ void seg_sums_plan(vt_ctx* ctx, const std::vector<int64_t>& voff, SegSums& s)
     h2d(ctx, s.pbase.p, pbase.data(), 8 * pbase.size());
 }
 
-void seg_tree_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b, double* d_out)
+void seg_tree_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b, double* d_out,
+                   const unsigned char* gact)
 {
     if (s.ng == 0) return;
     VT_LAUNCH(ctx, "seg_sum2_partial", k_seg_sum2_partial, (unsigned)s.nblocks, kSumBlock, 0, a, b,
               (const int64_t*)s.voff.p, (const int32_t*)s.bgraph.p, (const int32_t*)s.bsub.p,
-              (const int64_t*)s.chunk.p, s.part.p);
+              (const int64_t*)s.chunk.p, gact, s.part.p);
     VT_LAUNCH(ctx, "seg_sum2_final", k_seg_sum2_final, (unsigned)s.ng, kSumBlock, 0, (const double2*)s.part.p,
-              (const int64_t*)s.pbase.p, d_out);
+              (const int64_t*)s.pbase.p, gact, d_out);
 }
 
-void seg_seq_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b, double* d_out)
+void seg_seq_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b, double* d_out,
+                  const unsigned char* gact)
 {
     if (s.ng == 0) return;
     VT_LAUNCH(ctx, "seg_seq_sum2", k_seg_seq_sum2, grid_for(s.ng, 64, 4096), 64, 0, a, b, (const int64_t*)s.voff.p,
-              s.ng, d_out);
+              s.ng, gact, d_out);
 }
 
 // ---- scans and sorts (rocPRIM) --------------------------------------------------------

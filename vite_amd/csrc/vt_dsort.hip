@@ -7,12 +7,13 @@
 // at once.  They record no memset, allocate nothing once their scratch exists, and make no
 // host read, so a sequence of them can be captured and replayed.
 //
-// Sort: least-significant-digit radix sort, 8- or 9-bit digits, tiles of 2048 pairs.  Per pass a
-// histogram kernel counts each tile's digits (digit-major table: digit d of tile b at
-// d * ntiles + b), the table is scanned, and a scatter kernel places each pair at its digit's
-// offset for its tile plus its rank among the tile's pairs of that digit in input order.
-// Ranks come from wave ballots over the digit bits, so equal keys keep their input order
-// (stable), which is what the per-community event folds need.
+// Sort: least-significant-digit radix sort, 8- or 9-bit digits, tiles of 2048 pairs, each
+// block sorting a contiguous run of tiles.  Per pass a histogram kernel counts each block's
+// digits (digit-major table: digit d of block b at d * blocks + b), one block per digit scans
+// its row and totals it, and a scatter kernel places each pair at its digit's global offset
+// for its block plus its rank among the block's pairs of that digit in input order.  Ranks
+// come from wave ballots over the digit bits, so equal keys keep their input order (stable),
+// which is what the per-community event folds need.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -134,128 +135,194 @@ __global__ void __launch_bounds__(kScanBlock) k_dscan_down(const int32_t* __rest
     }
 }
 
-// Sizes of one sort: pairs, tiles and histogram entries (nbins per tile).
-__global__ void k_dsort_sizes(const int64_t* __restrict__ d_count, int64_t mult, int nbins, int64_t* __restrict__ sizes)
+// Sizes of one sort: [0] pairs, [1] tiles, [2] table entries (nbins per block), [3] blocks
+// that take part, [4] tiles per block (each block sorts a contiguous run of tiles).
+__global__ void k_dsort_sizes(const int64_t* __restrict__ d_count, int64_t mult, int nbins, int64_t grid,
+                              int64_t* __restrict__ sizes)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const int64_t n = *d_count * mult;
         const int64_t nt = ceil_div(n, kTile);
+        const int64_t per = ceil_div(nt, grid);
+        const int64_t nb = per ? ceil_div(nt, per) : 0;
         sizes[0] = n;
         sizes[1] = nt;
-        sizes[2] = nbins * nt;
+        sizes[2] = nbins * nb;
+        sizes[3] = nb;
+        sizes[4] = per;
     }
 }
 
 // The pass's digit: key bits [shift, shift + digit bits), cut at the sort's last bit (dmask).
 __device__ __forceinline__ uint32_t digit_of(uint32_t key, int shift, uint32_t dmask) { return (key >> shift) & dmask; }
 
-// Digit counts of each tile, digit-major (nbins = 256 or 512 bins).
+// Digit counts of each block's run of tiles, digit-major (digit d of block b at d * nb + b).
 __global__ void __launch_bounds__(kSortBlock) k_dsort_hist(const uint32_t* __restrict__ keys,
                                                          const int64_t* __restrict__ sizes, int shift, uint32_t dmask,
                                                          int nbins, int32_t* __restrict__ hist)
 {
     __shared__ int32_t s_hist[kMaxBins];
-    const int64_t n = sizes[0], nt = sizes[1];
-    for (int64_t b = blockIdx.x; b < nt; b += gridDim.x) {
-        for (int d = threadIdx.x; d < nbins; d += kSortBlock) s_hist[d] = 0;
-        __syncthreads();
-        for (int r = 0; r < kSortRounds; r++) {
-            const int64_t i = b * kTile + (int64_t)r * kSortBlock + threadIdx.x;
-            if (i < n) atomicAdd(&s_hist[digit_of(keys[i], shift, dmask)], 1);
-        }
-        __syncthreads();
-        for (int d = threadIdx.x; d < nbins; d += kSortBlock) hist[(int64_t)d * nt + b] = s_hist[d];
-        __syncthreads();
-    }
+    const int64_t n = sizes[0], nb = sizes[3], per = sizes[4];
+    const int64_t b = blockIdx.x;
+    if (b >= nb) return;
+    for (int d = threadIdx.x; d < nbins; d += kSortBlock) s_hist[d] = 0;
+    __syncthreads();
+    const int64_t lo = b * per * kTile;
+    const int64_t hi = std::min<int64_t>(n, lo + per * kTile);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kSortBlock) atomicAdd(&s_hist[digit_of(keys[i], shift, dmask)], 1);
+    __syncthreads();
+    for (int d = threadIdx.x; d < nbins; d += kSortBlock) hist[(int64_t)d * nb + b] = s_hist[d];
 }
 
-// Places the pairs of one tile in two steps.  First every pair gets its place in the tile
-// sorted by digit (the tile's pairs of smaller digits, then those of its digit in earlier
-// rounds, earlier waves of this round and earlier lanes of this wave) and is staged there in
-// LDS.  Then the staged tile is written out in that order: a pair at staged place p with
-// digit d goes to (scanned table entry of (d, tile)) + (p - first staged place of d), so the
-// pairs of one digit leave as one contiguous run and the writes coalesce.
+// One block per digit: the exclusive scan of the digit's block counts (offs, same layout as
+// hist) and the digit's total (dtot).
+__global__ void __launch_bounds__(kSortBlock) k_dsort_offsets(const int64_t* __restrict__ sizes,
+                                                            const int32_t* __restrict__ hist,
+                                                            int32_t* __restrict__ offs, int32_t* __restrict__ dtot)
+{
+    __shared__ int64_t s_wave[kWaves];
+    const int64_t nb = sizes[3];
+    const int d = blockIdx.x;
+    const int64_t per = ceil_div(nb, kSortBlock);  // at most kMaxGrid / kSortBlock = 8
+    const int64_t lo = (int64_t)threadIdx.x * per;
+    const int64_t hi = std::min<int64_t>(nb, lo + per);
+    const int32_t* h = hist + (int64_t)d * nb;
+    int64_t x = 0;
+    for (int64_t i = lo; i < hi; i++) x += h[i];
+    int64_t total = 0;
+    int64_t run = block_inclusive_scan(x, s_wave, &total) - x;
+    for (int64_t i = lo; i < hi; i++) {
+        const int32_t c = h[i];
+        offs[(int64_t)d * nb + i] = (int32_t)run;
+        run += c;
+    }
+    if (threadIdx.x == 0) dtot[d] = (int32_t)total;
+}
+
+// Each block places its run of tiles in order.  Per tile: every wave ranks its 512 pairs (8
+// steps of 64 lanes, in input order) among the wave's pairs of the same digit, from ballots
+// over the digit bits and a per-wave running count in LDS; the waves' counts give each
+// digit's place in the tile (a scan over digits) and each wave's place inside the digit; the
+// pairs are staged in LDS in (digit, input order) and written out with consecutive lanes on
+// consecutive positions.  The block's place for each digit starts at the digit's global
+// offset (all smaller digits, then the earlier blocks' pairs of the digit) and advances by
+// each tile's count.
 __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ vals,
                                                             const int64_t* __restrict__ sizes, int shift,
-                                                            uint32_t dmask, int dbits, const int32_t* __restrict__ hist,
+                                                            uint32_t dmask, int dbits,
                                                             const int32_t* __restrict__ offs,
+                                                            const int32_t* __restrict__ dtot,
                                                             uint32_t* __restrict__ keys_out,
                                                             uint32_t* __restrict__ vals_out)
 {
-    __shared__ int32_t s_run[kMaxBins];            // digit's pairs placed by earlier rounds
-    __shared__ int32_t s_wcnt[kWaves][kMaxBins];   // this round's pairs per wave and digit
-    __shared__ int32_t s_first[kMaxBins];          // first staged place of each digit
-    __shared__ int32_t s_out[kMaxBins];            // global place of each digit's first pair, minus s_first
+    constexpr int kSteps = kSortRounds;                 // 8 steps of 64 pairs per wave and tile
+    __shared__ int32_t s_wrun[kWaves][kMaxBins];        // per wave: running count, then place
+    __shared__ int32_t s_outt[kMaxBins];                // output position of the tile's staged index 0
     __shared__ int64_t s_wave[kWaves];
     __shared__ uint32_t s_key[kTile], s_val[kTile];
-    const int64_t n = sizes[0], nt = sizes[1];
+    const int64_t n = sizes[0], nb = sizes[3], per = sizes[4];
+    const int64_t b = blockIdx.x;
+    if (b >= nb) return;
     const int nbins = 1 << dbits;
     const int bpt = nbins / kSortBlock;  // bins per thread, consecutive: 1 or 2
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned long long below = (1ull << lane) - 1;
-    for (int64_t b = blockIdx.x; b < nt; b += gridDim.x) {
-        // the tile's digit counts, scanned: first staged place of each digit
-        int32_t cnt[kMaxBinsPerThread];
-        int32_t mine = 0;
-        for (int j = 0; j < bpt; j++) {
-            cnt[j] = hist[(int64_t)(threadIdx.x * bpt + j) * nt + b];
-            mine += cnt[j];
-        }
+
+    // the block's global place per digit: smaller digits' totals, then the earlier blocks
+    int64_t gbase[kMaxBinsPerThread];
+    {
+        int64_t mine = 0;
+        for (int j = 0; j < bpt; j++) mine += dtot[threadIdx.x * bpt + j];
         int64_t total = 0;
-        int32_t first = (int32_t)(block_inclusive_scan(mine, s_wave, &total) - mine);
+        int64_t run = block_inclusive_scan(mine, s_wave, &total) - mine;
         for (int j = 0; j < bpt; j++) {
             const int d = threadIdx.x * bpt + j;
-            s_first[d] = first;
-            s_out[d] = offs[(int64_t)d * nt + b] - first;
-            s_run[d] = 0;
-            first += cnt[j];
+            gbase[j] = run + offs[(int64_t)d * nb + b];
+            run += dtot[d];
+            for (int k = 0; k < kWaves; k++) s_wrun[k][d] = 0;
         }
-        const int64_t base = b * kTile;
+    }
+    __syncthreads();
+    const int64_t t_end = std::min<int64_t>(sizes[1], (b + 1) * per);
+    for (int64_t t = b * per; t < t_end; t++) {
+        const int64_t base = t * kTile;
         const int tile_n = (int)(n - base < kTile ? n - base : kTile);
-        for (int r = 0; r < kSortRounds; r++) {
-            for (int k = 0; k < kWaves; k++)
-                for (int d = threadIdx.x; d < nbins; d += kSortBlock) s_wcnt[k][d] = 0;
-            __syncthreads();
-            const int li = r * kSortBlock + threadIdx.x;
+        // (1) ranks inside the wave, in input order
+        uint32_t key[kSteps], val[kSteps];
+        int32_t rank[kSteps];
+        for (int r = 0; r < kSteps; r++) {
+            const int li = w * (kSteps * 64) + r * 64 + lane;
             const bool valid = li < tile_n;
-            uint32_t key = 0, val = 0;
-            if (valid) {
-                key = keys[base + li];
-                val = vals[base + li];
-            }
-            const uint32_t d = digit_of(key, shift, dmask);
-            // lanes of this wave holding the same digit
+            key[r] = valid ? keys[base + li] : 0u;
+            val[r] = valid ? vals[base + li] : 0u;
+        }
+        for (int r = 0; r < kSteps; r++) {
+            const int li = w * (kSteps * 64) + r * 64 + lane;
+            const bool valid = li < tile_n;
+            const uint32_t d = digit_of(key[r], shift, dmask);
             unsigned long long peers = __ballot(valid);
             for (int bit = 0; bit < dbits; bit++) {
                 const bool set = (d >> bit) & 1u;
                 const unsigned long long m = __ballot(set);
                 peers &= set ? m : ~m;
             }
-            const int rank = __popcll(peers & below);
-            if (valid && rank == 0) s_wcnt[w][d] = __popcll(peers);
-            __syncthreads();
-            if (valid) {
-                int place = s_first[d] + s_run[d] + rank;
-                for (int k = 0; k < w; k++) place += s_wcnt[k][d];
-                s_key[place] = key;
-                s_val[place] = val;
+            const int within = __popcll(peers & below);
+            const int32_t before = valid ? s_wrun[w][d] : 0;
+            rank[r] = before + within;
+            // every peer has read the count before its lowest lane moves it on (one wave, in order)
+            __builtin_amdgcn_wave_barrier();
+            if (valid && within == 0) s_wrun[w][d] = before + __popcll(peers);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        // (2) per digit: the tile's count, its first staged place, each wave's place
+        {
+            int32_t cnt[kMaxBinsPerThread];
+            int64_t mine = 0;
+            for (int j = 0; j < bpt; j++) {
+                const int d = threadIdx.x * bpt + j;
+                int32_t c = 0;
+                for (int k = 0; k < kWaves; k++) c += s_wrun[k][d];
+                cnt[j] = c;
+                mine += c;
             }
-            __syncthreads();
-            for (int dd = threadIdx.x; dd < nbins; dd += kSortBlock) {
-                int add = 0;
-                for (int k = 0; k < kWaves; k++) add += s_wcnt[k][dd];
-                s_run[dd] += add;
+            int64_t total = 0;
+            int32_t first = (int32_t)(block_inclusive_scan(mine, s_wave, &total) - mine);
+            for (int j = 0; j < bpt; j++) {
+                const int d = threadIdx.x * bpt + j;
+                int32_t run = first;
+                for (int k = 0; k < kWaves; k++) {
+                    const int32_t c = s_wrun[k][d];
+                    s_wrun[k][d] = run;
+                    run += c;
+                }
+                s_outt[d] = (int32_t)(gbase[j] - first);
+                gbase[j] += cnt[j];
+                first += cnt[j];
             }
         }
         __syncthreads();
+        // (3) stage in (digit, input order)
+        for (int r = 0; r < kSteps; r++) {
+            const int li = w * (kSteps * 64) + r * 64 + lane;
+            if (li < tile_n) {
+                const int place = s_wrun[w][digit_of(key[r], shift, dmask)] + rank[r];
+                s_key[place] = key[r];
+                s_val[place] = val[r];
+            }
+        }
+        __syncthreads();
+        // (4) write out; (5) clear the running counts for the next tile (not read in (4))
         for (int p = threadIdx.x; p < tile_n; p += kSortBlock) {
-            const uint32_t key = s_key[p];
-            const int64_t pos = (int64_t)s_out[digit_of(key, shift, dmask)] + p;
-            keys_out[pos] = key;
+            const uint32_t k = s_key[p];
+            const int64_t pos = (int64_t)s_outt[digit_of(k, shift, dmask)] + p;
+            keys_out[pos] = k;
             vals_out[pos] = s_val[p];
         }
-        __syncthreads();  // the staging arrays are refilled for the next tile
+        for (int j = 0; j < bpt; j++)
+            for (int k = 0; k < kWaves; k++) s_wrun[k][threadIdx.x * bpt + j] = 0;
+        __syncthreads();  // staging, s_outt and the cleared counts are reused by the next tile
     }
 }
 
@@ -287,10 +354,11 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
     // (the move events of up to 2^27 vertices) in three passes instead of four
     const int dbits = bits > 24 ? kMaxDigitBits : 8;
     const int nbins = 1 << dbits;
-    int64_t* sizes = (int64_t*)scratch(ctx, "dsort_sizes", 8 * 4);
-    int32_t* hist = (int32_t*)scratch(ctx, "dsort_hist", 4 * (size_t)nbins * (size_t)nt_max);
-    int32_t* offs = (int32_t*)scratch(ctx, "dsort_offs", 4 * (size_t)nbins * (size_t)nt_max);
-    VT_LAUNCH(ctx, "dsort_sizes", k_dsort_sizes, 1, 64, 0, d_count, mult, nbins, sizes);
+    int64_t* sizes = (int64_t*)scratch(ctx, "dsort_sizes", 8 * 8);
+    int32_t* hist = (int32_t*)scratch(ctx, "dsort_hist", 4 * (size_t)nbins * grid);
+    int32_t* offs = (int32_t*)scratch(ctx, "dsort_offs", 4 * (size_t)nbins * grid);
+    int32_t* dtot = (int32_t*)scratch(ctx, "dsort_dtot", 4 * (size_t)nbins);
+    VT_LAUNCH(ctx, "dsort_sizes", k_dsort_sizes, 1, 64, 0, d_count, mult, nbins, (int64_t)grid, sizes);
     bool alt = false;
     for (int shift = 0; shift < bits; shift += dbits) {
         const uint32_t dmask = bits - shift >= dbits ? (uint32_t)nbins - 1u : (1u << (bits - shift)) - 1u;
@@ -300,10 +368,11 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
         uint32_t* vo = alt ? vals : vals_alt;
         VT_LAUNCH(ctx, "dsort_hist", k_dsort_hist, grid, kSortBlock, 0, (const uint32_t*)ki,
                   (const int64_t*)sizes, shift, dmask, nbins, hist);
-        dev_scan_i32(ctx, hist, offs, sizes + 2, (int64_t)nbins * nt_max, nullptr, "dsort_part");
+        VT_LAUNCH(ctx, "dsort_offsets", k_dsort_offsets, (unsigned)nbins, kSortBlock, 0, (const int64_t*)sizes,
+                  (const int32_t*)hist, offs, dtot);
         VT_LAUNCH(ctx, "dsort_scatter", k_dsort_scatter, grid, kSortBlock, 0, (const uint32_t*)ki,
-                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, dbits, (const int32_t*)hist,
-                  (const int32_t*)offs, ko, vo);
+                  (const uint32_t*)vi, (const int64_t*)sizes, shift, dmask, dbits, (const int32_t*)offs,
+                  (const int32_t*)dtot, ko, vo);
         alt = !alt;
     }
     return alt;

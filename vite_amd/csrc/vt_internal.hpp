@@ -213,8 +213,11 @@ struct SegSums {
     DBuf<double2> part;
 };
 void seg_sums_plan(vt_ctx* ctx, const std::vector<int64_t>& voff, SegSums& s);
-void seg_tree_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b_sq, double* d_out);
-void seg_seq_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b_sq, double* d_out);
+// gact (optional): graphs whose flag is 0 are skipped and their sums left as they were.
+void seg_tree_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b_sq, double* d_out,
+                   const unsigned char* gact = nullptr);
+void seg_seq_sum2(vt_ctx* ctx, const SegSums& s, const double* a, const double* b_sq, double* d_out,
+                  const unsigned char* gact = nullptr);
 void exclusive_scan_i64(vt_ctx* ctx, const int64_t* in, int64_t* out, int64_t n);
 void exclusive_scan_i32(vt_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 // Stable radix sort of (key, value) pairs on key bits [0, bits).

@@ -1658,10 +1658,14 @@ __global__ void k_narrow_ids(const int64_t* __restrict__ src, int64_t n, int32_t
         dst[i] = (int32_t)src[i];
 }
 
+// Records of vertices 0..n-1, or of the listed vertices live[0..n-1] (a batch's graphs still
+// iterating: the sweep reads no other records).
 __global__ void k_pack_records(const int64_t* __restrict__ curr, const int64_t* __restrict__ csize,
-                               const double* __restrict__ cdeg, int64_t n, VRec* __restrict__ rec)
+                               const double* __restrict__ cdeg, int64_t n, const int32_t* __restrict__ live,
+                               VRec* __restrict__ rec)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = live ? live[p] : p;
         VRec r;
         r.comm = (int32_t)curr[i];
         r.size = (int32_t)csize[i];
@@ -1682,13 +1686,33 @@ __global__ void k_seg_constant(const double* __restrict__ sums, int32_t ng, doub
 // vertex that does not move, or whose graph has left the phase, gets key `none`, which sorts
 // after every community.  A stable sort then gives each community its events in vertex
 // order, as the one-graph path does with a compacted mover list.
-// A batch vertex moves when its graph is still iterating and its target differs.
-__global__ void k_batch_flags(int64_t nv, const int64_t* __restrict__ target, const int64_t* __restrict__ curr,
-                              const int32_t* __restrict__ gid, const unsigned char* __restrict__ gact,
-                              int32_t* __restrict__ flags)
+// A batch vertex moves when its graph is still iterating and its target differs.  Flags are
+// per position of the live list (the vertices of the graphs iterating at the last relist, in
+// vertex order), so the scan and the events follow vertex order.
+__global__ void k_batch_flags(int64_t n, const int32_t* __restrict__ live, const int64_t* __restrict__ target,
+                              const int64_t* __restrict__ curr, const int32_t* __restrict__ gid,
+                              const unsigned char* __restrict__ gact, int32_t* __restrict__ flags)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = live[p];
+        flags[p] = (gact[gid[v]] && target[v] != curr[v]) ? 1 : 0;
+    }
+}
+
+// The live list: the vertices of the graphs still iterating, in vertex order (a flag pass,
+// a scan of the flags, then each flagged vertex written at its scanned place).
+__global__ void k_live_flags(int64_t nv, const int32_t* __restrict__ gid, const unsigned char* __restrict__ gact,
+                             int32_t* __restrict__ flags)
 {
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x)
-        flags[v] = (gact[gid[v]] && target[v] != curr[v]) ? 1 : 0;
+        flags[v] = gact[gid[v]] ? 1 : 0;
+}
+
+__global__ void k_live_scatter(int64_t nv, const int32_t* __restrict__ flags, const int32_t* __restrict__ pos,
+                               int32_t* __restrict__ live)
+{
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x)
+        if (flags[v]) live[pos[v]] = (int32_t)v;
 }
 
 // The end of the reference loop body per graph (louvain.cpp:540-577): currMod from the
@@ -1721,14 +1745,20 @@ __global__ void k_batch_decide(int32_t ng, const double* __restrict__ sums, cons
     }
 }
 
-// past <- curr <- target for the graphs that continue (louvain.cpp:565-570).
-__global__ void k_batch_rotate(int64_t nv, const int32_t* __restrict__ gid, const unsigned char* __restrict__ grot,
-                               int64_t* __restrict__ past, int64_t* __restrict__ curr, const int64_t* __restrict__ target)
+// past <- curr <- target for the graphs that continue (louvain.cpp:565-570), over the live
+// list; the 32-bit copy of currComm the sweep reads (curr32) is kept equal to curr here.
+__global__ void k_batch_rotate(int64_t n, const int32_t* __restrict__ live, const int32_t* __restrict__ gid,
+                               const unsigned char* __restrict__ grot, int64_t* __restrict__ past,
+                               int64_t* __restrict__ curr, const int64_t* __restrict__ target,
+                               int32_t* __restrict__ curr32)
 {
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = live[p];
         if (!grot[gid[v]]) continue;
+        const int64_t t = target[v];
         past[v] = curr[v];
-        curr[v] = target[v];
+        curr[v] = t;
+        if (curr32) curr32[v] = (int32_t)t;
     }
 }
 
@@ -1786,6 +1816,11 @@ struct Phase {
     DBuf<int32_t> tail32, curr32;
     DBuf<double> wgt;
     DBuf<VRec> rec;  // community records, rebuilt before every sweep when the tuning asks
+    // A batch narrows currComm once and keeps curr32 current in its rotation (narrow_once);
+    // its records are packed for the live list only (pack_live, pack_n) when set.
+    bool narrow_once = false, narrowed = false;
+    const int32_t* pack_live = nullptr;
+    int64_t pack_n = 0;
 
     Phase(vt_ctx* c, const vt_graph* gr) : ctx(c), g(gr), nv(gr->nv) {}
 
@@ -1801,11 +1836,15 @@ struct Phase {
                 VT_LAUNCH(ctx, "split_edges", k_split_edges, grid_for(g->ne, 256, 16384), 256, 0,
                           (const DEdge*)g->edges, g->ne, tail32.p, wgt.p);
         }
-        if (nv) VT_LAUNCH(ctx, "narrow_curr", k_narrow_ids, grid_for(nv, 256, 8192), 256, 0, (const int64_t*)curr.p, nv, curr32.p);
+        if (nv && !(narrow_once && narrowed))
+            VT_LAUNCH(ctx, "narrow_curr", k_narrow_ids, grid_for(nv, 256, 8192), 256, 0, (const int64_t*)curr.p, nv, curr32.p);
+        narrowed = true;
         if (ctx->tuning.vertex_records && nv) {
             if (!rec.p) rec = DBuf<VRec>(ctx, (size_t)nv);
-            VT_LAUNCH(ctx, "pack_records", k_pack_records, grid_for(nv, 256, 8192), 256, 0, (const int64_t*)curr.p,
-                      (const int64_t*)csize.p, (const double*)cdeg.p, nv, rec.p);
+            const int64_t n = pack_live ? pack_n : nv;
+            if (n)
+                VT_LAUNCH(ctx, "pack_records", k_pack_records, grid_for(n, 256, 8192), 256, 0, (const int64_t*)curr.p,
+                          (const int64_t*)csize.p, (const double*)cdeg.p, n, pack_live, rec.p);
         }
     }
 
@@ -2301,9 +2340,9 @@ double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cv
 // iteration count and partition equal those of run_phase on that graph alone.  The
 // partition is left on `u` (u->part) for the rebuild.
 //
-// With `use_graph`, the iteration after the first is replayed from two captured HIP graphs
-// (local move + events; folds + sums + decisions + rotation) around the event sort, which
-// stays outside capture because the library sort clears its buffers with memsets.
+// With `use_graph`, every round is replayed from one captured HIP graph (sweep, movers' scan,
+// device sort of the events, folds, sums, decisions, rotation), except round 0 and the round
+// after each relist, which run directly while the host captures the next one.
 void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,
                  const std::vector<double>& thresh, int flags, bool use_graph, std::vector<double>& q_out,
                  std::vector<int32_t>& iters_out, int* rounds_out, int* relists_out)
@@ -2316,8 +2355,10 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     DBuf<unsigned char> gact(ctx, ng), grot(ctx, ng);
     DBuf<int32_t> giters(ctx, ng);
     // [0] graphs still iterating, [1] their vertices (set by each round's decisions);
-    // [2] nv, the length of the movers' flag scan; [3] movers (that scan's total)
-    DBuf<unsigned long long> status(ctx, 4);
+    // [2] the live list's length (the movers' flag scan runs over it); [3] movers (that
+    // scan's total); [4] nv
+    DBuf<unsigned long long> status(ctx, 5);
+    DBuf<int32_t> live(ctx, (size_t)std::max<int64_t>(nv, 1));
     ph.gid = d_gid;
     ph.gact = gact.p;
     ph.gconst = gconst.p;
@@ -2335,7 +2376,8 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     {
         std::vector<unsigned char> ones((size_t)ng, 1);
         h2d(ctx, gact.p, ones.data(), (size_t)ng);
-        const unsigned long long st0[4] = {(unsigned long long)ng, (unsigned long long)nv, (unsigned long long)nv, 0};
+        const unsigned long long st0[5] = {(unsigned long long)ng, (unsigned long long)nv, (unsigned long long)nv, 0,
+                                           (unsigned long long)nv};
         h2d(ctx, status.p, st0, sizeof(st0));
     }
     ph.build_lists(nullptr, 1, false);
@@ -2351,8 +2393,29 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     int32_t* mpos = (int32_t*)scratch(ctx, "bmv_pos", 4 * (size_t)std::max<int64_t>(nv, 1));
     const uint32_t none = (uint32_t)nv;
     const int bits = (int)bits_for(std::max<int64_t>(nv, 1));
-    int64_t* d_nv = (int64_t*)(status.p + 2);
+    int64_t* d_nlive = (int64_t*)(status.p + 2);
     int64_t* d_movers = (int64_t*)(status.p + 3);
+    const int64_t* d_nv = (const int64_t*)(status.p + 4);
+    // The live list from the graphs' flags as they stand once the queued work has run; its
+    // length is read back (the rounds captured after it take it as their launch size).
+    int64_t nlive = nv;
+    auto build_live = [&]() {
+        if (!nv) {
+            nlive = 0;
+            return;
+        }
+        VT_LAUNCH(ctx, "live_flags", k_live_flags, grid_for(nv, 256, 8192), 256, 0, nv, d_gid,
+                  (const unsigned char*)gact.p, mflags);
+        dev_scan_i32(ctx, mflags, mpos, d_nv, nv, d_nlive, "blive_part");
+        VT_LAUNCH(ctx, "live_scatter", k_live_scatter, grid_for(nv, 256, 8192), 256, 0, nv, (const int32_t*)mflags,
+                  (const int32_t*)mpos, live.p);
+        d2h(ctx, &nlive, d_nlive, 8);
+        sync(ctx);
+    };
+    build_live();
+    ph.narrow_once = true;  // k_batch_rotate keeps curr32 equal to curr from here on
+    ph.pack_live = live.p;
+    ph.pack_n = nlive;
 
     // One round of the reference loop body (louvain.cpp:498-577) for every graph still
     // iterating, with no host read: the sweep; the movers' flags and their scan (whose total
@@ -2362,22 +2425,29 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     // step, so a round after the last graph has left changes nothing.
     auto round = [&]() {
         ph.sweep(ph.sweeps[0]);
-        VT_LAUNCH(ctx, "batch_flags", k_batch_flags, grid_for(nv, 256, 8192), 256, 0, nv, (const int64_t*)ph.target.p,
-                  (const int64_t*)ph.curr.p, d_gid, (const unsigned char*)gact.p, mflags);
-        dev_scan_i32(ctx, mflags, mpos, d_nv, nv, d_movers, "bmv_part");
-        VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nv, 256, 8192), 256, 0, (const int32_t*)nullptr, nv,
-                  (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p, (const int64_t*)ph.curr.p,
-                  k0, v0);
+        if (nlive) {
+            VT_LAUNCH(ctx, "batch_flags", k_batch_flags, grid_for(nlive, 256, 8192), 256, 0, nlive,
+                      (const int32_t*)live.p, (const int64_t*)ph.target.p, (const int64_t*)ph.curr.p, d_gid,
+                      (const unsigned char*)gact.p, mflags);
+            dev_scan_i32(ctx, mflags, mpos, d_nlive, nlive, d_movers, "bmv_part");
+            VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nlive, 256, 8192), 256, 0, (const int32_t*)live.p,
+                      nlive, (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p,
+                      (const int64_t*)ph.curr.p, k0, v0);
+        } else {
+            dzero(ctx, d_movers, 8);  // every graph has left: no events
+        }
         const bool alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, d_movers, 2, nev, bits);
         ph.fold_events(alt ? k1 : k0, alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p, d_movers);
-        if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
-        else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p);
+        if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p, (const unsigned char*)gact.p);
+        else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p, (const unsigned char*)gact.p);
         dzero(ctx, status.p, 16);
         VT_LAUNCH(ctx, "batch_decide", k_batch_decide, grid_for(ng, 256, 64), 256, 0, ng, (const double*)gsums.p,
                   (const double*)gconst.p, (const double*)gthr.p, (const double*)glow.p, gact.p, grot.p, gprev.p,
                   giters.p, 0, (const int64_t*)seg.voff.p, status.p);
-        VT_LAUNCH(ctx, "batch_rotate", k_batch_rotate, grid_for(nv, 256, 8192), 256, 0, nv, d_gid,
-                  (const unsigned char*)grot.p, ph.past.p, ph.curr.p, (const int64_t*)ph.target.p);
+        if (nlive)
+            VT_LAUNCH(ctx, "batch_rotate", k_batch_rotate, grid_for(nlive, 256, 8192), 256, 0, nlive,
+                      (const int32_t*)live.p, d_gid, (const unsigned char*)grot.p, ph.past.p, ph.curr.p,
+                      (const int64_t*)ph.target.p, ph.curr32.p);
     };
 
     hipGraphExec_t gexec = nullptr;
@@ -2420,29 +2490,33 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
         for (auto& e : ev) VT_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         int64_t listed = nv;
         for (int r = 0;; r++) {
-            // round 0 runs directly: it allocates the scratch the captured rounds reuse
-            if (use_graph && r >= 1 && !gexec) capture();
+            // A round without a captured graph (round 0, which allocates the scratch the
+            // captured rounds reuse, and the round after a relist) runs directly, and the
+            // next round is captured while it runs on the device.
+            const bool direct = !gexec;
             if (gexec) VT_HIP(ctx, hipGraphLaunch(gexec, ctx->stream));
             else round();
             unsigned long long* slot = pin + 2 * (r & 1);
             VT_HIP(ctx, hipMemcpyAsync(slot, status.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                        ctx->stream));
             VT_HIP(ctx, hipEventRecord(ev[r & 1], ctx->stream));
+            if (use_graph && direct) capture();
             if (r == 0) continue;
             VT_HIP(ctx, hipEventSynchronize(ev[(r - 1) & 1]));
             const unsigned long long* prev = pin + 2 * ((r - 1) & 1);
-            const int64_t active = (int64_t)prev[0], live = (int64_t)prev[1];
+            const int64_t active = (int64_t)prev[0], live_nv = (int64_t)prev[1];
             if (active == 0) {
                 rounds = r;  // rounds 0 .. r-1 had graphs iterating; round r was a no-op
                 break;
             }
-            if (2 * live < listed) {
+            if (2 * live_nv < listed) {
                 // drop the finished graphs' vertices from the sweep lists (rebuilt after the
                 // queued round, from the flags as they stand then); the round is captured anew
-                listed = live;
+                listed = live_nv;
                 relists++;
                 ph.build_lists(nullptr, 1, false);
-                sync(ctx);  // the queued round, which replayed the old capture, has finished
+                build_live();  // ends with a sync: the queued round, which replayed the old capture, has finished
+                ph.pack_n = nlive;
                 if (gexec) (void)hipGraphExecDestroy(gexec);
                 gexec = nullptr;
             }
