@@ -119,12 +119,15 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 // Positions of community c's concatenated row -> CSR indices (moff: each member's start in
-// its community's concatenated row; mrow: its row start in the CSR).  Up to 64 members,
-// lane i holds member i's two starts and lanes find their member with shuffles; larger
-// communities search moff.  c is the same in every lane, so the community's bounds come in
-// through scalar loads.
+// its community's concatenated row; mrow: its row start in the CSR).  Up to 64 members
+// (whose rows total under 2^31 entries), lane i holds member i's start relative to the
+// community's (32 bits) and its row start minus its start, and lanes find their member by a
+// binary search over shuffles that starts at the member count's highest power of two;
+// larger communities search moff.  c is the same in every lane, so the community's bounds
+// come in through scalar loads.
 struct MemberRows {
-    int64_t m0, m1, s0, s1, my_s = INT64_MAX, my_row = 0;
+    int64_t m0, m1, s0, s1, my_base = 0;
+    int32_t my_rel = INT32_MAX, top = 0;
     bool small;
 
     __device__ MemberRows(const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff,
@@ -134,10 +137,15 @@ struct MemberRows {
         m1 = cstart[c + 1];
         s0 = moff[m0];
         s1 = moff[m1];
-        small = m1 - m0 <= 64;
-        if (small && lane < m1 - m0) {
-            my_s = moff[m0 + lane];
-            my_row = mrow[m0 + lane];
+        small = m1 - m0 <= 64 && s1 - s0 < INT32_MAX;
+        if (small) {
+            const int nm = (int)(m1 - m0);
+            top = nm > 1 ? 1 << (31 - __builtin_clz((unsigned)(nm - 1))) : 0;
+            if (lane < nm) {
+                const int64_t ms = moff[m0 + lane];
+                my_rel = (int32_t)(ms - s0);
+                my_base = mrow[m0 + lane] - ms;
+            }
         }
     }
 
@@ -146,14 +154,15 @@ struct MemberRows {
     {
         if (small) {
             const int nm = (int)(m1 - m0);
+            const int64_t q64 = p - s0;
+            const int32_t q = q64 < INT32_MAX ? (int32_t)q64 : INT32_MAX;
             int lo = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
+            for (int step = top; step > 0; step >>= 1) {
                 const int cand = lo + step;
-                const int64_t cs = __shfl(my_s, cand < 64 ? cand : 63);
-                if (cand < nm && cs <= p) lo = cand;
+                const int32_t cs = __shfl(my_rel, cand < 64 ? cand : 63);
+                if (cand < nm && cs <= q) lo = cand;
             }
-            return __shfl(my_row, lo) + (p - __shfl(my_s, lo));
+            return p + __shfl(my_base, lo);
         }
         if (p >= s1) return 0;
         int64_t lo = m0, hi = m1 - 1;  // the member whose row holds position p
@@ -167,6 +176,10 @@ struct MemberRows {
 };
 
 constexpr int kCfWaves = 4;
+// The fold's blocks are one wave each: a block holds its LDS tables until its last wave
+// ends, and the waves of a block take communities of very different sizes, so 4-wave blocks
+// left about half of the resident-wave slots the LDS allows idle.
+constexpr int kFoldWaves = 1;
 constexpr int kCfLog = 8;
 constexpr int kCfCap = 1 << kCfLog;       // table slots per wave
 constexpr int kCfMaxKeys = 128;           // distinct keys a community may hold in the table
@@ -176,17 +189,17 @@ constexpr uint32_t kNoKey = 0xffffffffu;
 // Per community c: its coarse row staged at rkey/rval[s0, s0 + rowlen[c]), with s0 its
 // offset in the concatenated rows; or, with more distinct keys than the table holds,
 // rowlen[c] = 0 and xcnt[c] = its entry count, for the sort path.
-__global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
+__global__ void __launch_bounds__(64 * kFoldWaves) k_comm_fold(
     int64_t nc, const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow,
     const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
     uint32_t* __restrict__ rkey, double* __restrict__ rval, int64_t* __restrict__ rowlen, int64_t* __restrict__ xcnt)
 {
-    __shared__ uint32_t tk[kCfWaves][kCfCap];
-    __shared__ double tv[kCfWaves][kCfCap];
-    __shared__ unsigned long long tm[kCfWaves][kCfCap];
-    __shared__ uint32_t tl[kCfWaves][kCfList];  // keys in insertion order
-    __shared__ uint16_t ts[kCfWaves][kCfList];  // and their slots
-    __shared__ double wst[kCfWaves][65];        // [0] = -0.0, then the chunk's weights
+    __shared__ uint32_t tk[kFoldWaves][kCfCap];
+    __shared__ double tv[kFoldWaves][kCfCap];
+    __shared__ unsigned long long tm[kFoldWaves][kCfCap];
+    __shared__ uint32_t tl[kFoldWaves][kCfList];  // keys in insertion order
+    __shared__ uint16_t ts[kFoldWaves][kCfList];  // and their slots
+    __shared__ double wst[kFoldWaves][65];        // [0] = -0.0, then the chunk's weights
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned long long below = (1ull << lane) - 1;
     for (int i = lane; i < kCfCap; i += 64) {
@@ -196,7 +209,7 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_comm_fold(
     }
     if (lane == 0) wst[w][0] = -0.0;
     wave_lds_sync();
-    for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
+    for (int64_t c = (int64_t)blockIdx.x * kFoldWaves + w; c < nc; c += (int64_t)gridDim.x * kFoldWaves) {
         const MemberRows mr(cstart, moff, mrow, c, lane);
         int nd = 0;  // distinct external keys so far (the table never holds more than kCfList)
         double self = -0.0;  // the internal entries' fold, the same in every lane
@@ -553,7 +566,7 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     dzero(ctx, rowlen.p + nc, 8);
     dzero(ctx, xcnt.p + nc, 8);
     if (nc)
-        VT_LAUNCH(ctx, "rb_comm_fold", k_comm_fold, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc,
+        VT_LAUNCH(ctx, "rb_comm_fold", k_comm_fold, grid_for(nc, kFoldWaves, 262144), 64 * kFoldWaves, 0, nc,
                   (const int64_t*)cstart.p, (const int64_t*)moff.p, (const int64_t*)mrow.p, (const DEdge*)g->edges, newid,
                   rkey.p, rval.p, rowlen.p, xcnt.p);
     exclusive_scan_i64(ctx, rowlen.p, out_offs.p, nc + 1);
