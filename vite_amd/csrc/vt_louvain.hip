@@ -1426,6 +1426,16 @@ __global__ void k_mover_flags(const int32_t* __restrict__ order, int64_t n, cons
     }
 }
 
+__global__ void k_store_i64(int64_t* __restrict__ p, int64_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+__global__ void k_add_i64(int64_t* __restrict__ acc, const int64_t* __restrict__ x)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *x;
+}
+
 __global__ void k_scan_total(const int32_t* __restrict__ flags, const int32_t* __restrict__ pos, int64_t n,
                              int64_t* __restrict__ total)
 {
@@ -1490,6 +1500,10 @@ __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* _
             *long_runs = 1;
             continue;
         }
+        // the community's counters, read before the run so their (random) loads overlap
+        // the run's; no other run touches community c
+        const double deg0 = cdeg[c];
+        const int64_t size0 = csize[c];
         double acc = 0.0;
         int64_t cnt = 0;
         int64_t k = j;
@@ -1506,8 +1520,8 @@ __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* _
             acc = acc + delta[k];
             cnt += dir[k];
         }
-        cdeg[c] = cdeg[c] + acc;
-        csize[c] += cnt;
+        cdeg[c] = deg0 + acc;
+        csize[c] = size0 + cnt;
     }
 }
 
@@ -1796,7 +1810,7 @@ struct Phase {
     DBuf<int32_t> lists;     // vertices grouped by (color, degree bucket), vertex order inside
     DBuf<int32_t> corder;    // vertices by color, vertex order inside (event order for color modes)
     DBuf<double> scal;       // [0..1] sums, [2] constant, [3] q
-    DBuf<int64_t> iscal;     // [0] mover total
+    DBuf<int64_t> iscal;     // [0] movers of the last apply, [1] of the iteration, [2] scan length
     DBuf<unsigned long long> ucount;
     std::vector<ColorSweep> sweeps;      // per color (one entry in plain/vorder)
     std::vector<int64_t> color_ptr;      // colorPtr (louvain.cpp:647-661)
@@ -2135,31 +2149,38 @@ struct Phase {
         }
     }
 
-    // Folds the sweep's moves into localCinfo in processing order; returns the mover count.
-    int64_t apply_moves(const int32_t* order, int64_t n, double* dst_cdeg, int64_t* dst_csize)
+    // Folds the sweep's moves into localCinfo in processing order, with no host read: the
+    // mover count stays on the device (the scan's total sizes the event sort and folds) and
+    // is added to iscal[1], the iteration's movers, which the host reads with Q.
+    void apply_moves(const int32_t* order, int64_t n, double* dst_cdeg, int64_t* dst_csize)
     {
-        if (n == 0) return 0;
+        if (n == 0) return;
         int32_t* flags = (int32_t*)scratch(ctx, "mv_flags", 4 * (size_t)n);
         int32_t* pos = (int32_t*)scratch(ctx, "mv_pos", 4 * (size_t)n);
+        int64_t* d_movers = iscal.p;
+        int64_t* d_n = iscal.p + 2;
+        VT_LAUNCH(ctx, "store_count", k_store_i64, 1, 64, 0, d_n, n);
         VT_LAUNCH(ctx, "mover_flags", k_mover_flags, grid_for(n, 256, 8192), 256, 0, order, n, target.p, curr.p,
                   et ? active.p : nullptr, flags);
-        exclusive_scan_i32(ctx, flags, pos, n);
-        VT_LAUNCH(ctx, "scan_total", k_scan_total, 1, 64, 0, (const int32_t*)flags, (const int32_t*)pos, n, iscal.p);
-        int64_t movers = 0;
-        d2h(ctx, &movers, iscal.p, 8);
-        if (movers == 0) return 0;
-        const int64_t ne = 2 * movers;
+        dev_scan_i32(ctx, flags, pos, d_n, n, d_movers, "mv_part");
+        const int64_t ne = 2 * n;  // events: at most two per vertex
         uint32_t* k0 = (uint32_t*)scratch(ctx, "ev_k0", 4 * (size_t)ne);
         uint32_t* v0 = (uint32_t*)scratch(ctx, "ev_v0", 4 * (size_t)ne);
         uint32_t* k1 = (uint32_t*)scratch(ctx, "ev_k1", 4 * (size_t)ne);
         uint32_t* v1 = (uint32_t*)scratch(ctx, "ev_v1", 4 * (size_t)ne);
         VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(n, 256, 8192), 256, 0, order, n, (const int32_t*)flags,
                   (const int32_t*)pos, target.p, curr.p, k0, v0);
-        bool alt = false;
-        sort_pairs_u32(ctx, k0, v0, ne, (int)bits_for(nv > 1 ? nv - 1 : 1), k1, v1, &alt);
+        const bool alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, d_movers, 2, ne, (int)bits_for(nv > 1 ? nv - 1 : 1));
         fold_events(alt ? k1 : k0, alt ? v1 : v0, ne, 0xFFFFFFFFu, (double*)scratch(ctx, "ev_d", 8 * (size_t)ne),
-                    (int8_t*)scratch(ctx, "ev_s", (size_t)ne), dst_cdeg, dst_csize);
-        return movers;
+                    (int8_t*)scratch(ctx, "ev_s", (size_t)ne), dst_cdeg, dst_csize, d_movers);
+        VT_LAUNCH(ctx, "count_movers", k_add_i64, 1, 64, 0, iscal.p + 1, (const int64_t*)d_movers);
+    }
+
+    int64_t iteration_movers()
+    {
+        int64_t m = 0;
+        d2h(ctx, &m, iscal.p + 1, 8);
+        return m;
     }
 
     // localCinfo += the sorted events' folds, per community in processing order.
@@ -2257,12 +2278,12 @@ void phase_step(vt_ctx* ctx, vt_phase* st)
     const int64_t nv = ph.nv;
     st->numIters++;
     if (ph.et) dzero(ctx, ph.ucount.p, 8);
-    int64_t moves = 0;
+    dzero(ctx, ph.iscal.p + 1, 8);  // the iteration's movers
     if (st->mode == VT_MODE_COLOR) {
         for (int64_t ci = 0; ci < p.num_color; ci++) {
             ph.sweep(ph.sweeps[ci]);
             const int64_t a0 = ph.color_ptr[ci], a1 = ph.color_ptr[ci + 1];
-            moves += ph.apply_moves(ph.corder.p + a0, a1 - a0, ph.cdeg.p, ph.csize.p);
+            ph.apply_moves(ph.corder.p + a0, a1 - a0, ph.cdeg.p, ph.csize.p);
         }
     } else {
         ph.sweep(ph.sweeps[0]);
@@ -2273,17 +2294,17 @@ void phase_step(vt_ctx* ctx, vt_phase* st)
         if (vc >= 90) {  // ET_CUTOFF compared with a count (louvain.cpp:116-121)
             st->currMod = -1.0;
             ctx->trace_q.push_back(NAN);
-            ctx->trace_moves.push_back(moves);
+            ctx->trace_moves.push_back(ph.iteration_movers());
             st->done = true;
             return;
         }
     }
-    if (st->mode == VT_MODE_PLAIN) moves = ph.apply_moves(nullptr, nv, ph.cdeg.p, ph.csize.p);
-    else if (st->mode == VT_MODE_VORDER) moves = ph.apply_moves(ph.corder.p, nv, ph.cdeg.p, ph.csize.p);
+    if (st->mode == VT_MODE_PLAIN) ph.apply_moves(nullptr, nv, ph.cdeg.p, ph.csize.p);
+    else if (st->mode == VT_MODE_VORDER) ph.apply_moves(ph.corder.p, nv, ph.cdeg.p, ph.csize.p);
 
     st->currMod = ph.modularity(p.flags);
     ctx->trace_q.push_back(st->currMod);
-    ctx->trace_moves.push_back(moves);
+    ctx->trace_moves.push_back(ph.iteration_movers());
     if ((st->currMod - st->prevMod) < p.thresh) { st->done = true; return; }
     st->prevMod = st->currMod;
     if (st->prevMod < p.lower) st->prevMod = p.lower;
