@@ -514,7 +514,14 @@ __global__ void k_max_degree(const int64_t* __restrict__ offs, int64_t nv, unsig
         const int64_t x = __shfl_xor(m, o);
         m = x > m ? x : m;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)m);
+    // one atomic per block: every block's atomics go to the one address
+    __shared__ int64_t s_max[256 / 64];
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = s_max[k] > m ? s_max[k] : m;
+        atomicMax(out, (unsigned long long)m);
+    }
 }
 
 // Validation: offsets monotone from 0 to ne, tails in range.  out[0] counts bad rows,
@@ -540,7 +547,7 @@ void graph_finalize(vt_ctx* ctx, vt_graph* g)
     unsigned long long* d = (unsigned long long*)scratch(ctx, "small_u64", 64);
     dzero(ctx, d, 8);
     if (g->nv)
-        VT_LAUNCH(ctx, "max_degree", k_max_degree, grid_for(g->nv, 256, 4096), 256, 0, g->offs, g->nv, d);
+        VT_LAUNCH(ctx, "max_degree", k_max_degree, grid_for(g->nv, 256, 512), 256, 0, g->offs, g->nv, d);
     unsigned long long h = 0;
     d2h(ctx, &h, d, 8);
     g->max_degree = (int64_t)h;

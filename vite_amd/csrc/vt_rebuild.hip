@@ -29,6 +29,9 @@ __global__ void k_first_vertex(const int64_t* __restrict__ cvect, int64_t nv, un
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nv; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = cvect[i];
         if (c < 0 || c >= nv) { atomicAdd(bad, 1ull); continue; }
+        // vertex i - 1 in the same community: i is not the community's first vertex (on
+        // graphs numbered by locality most vertices follow one of their own community)
+        if (i > 0 && cvect[i - 1] == c) continue;
         atomicMin(&first[c], (unsigned long long)i);
     }
 }
@@ -88,10 +91,26 @@ __global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __
 // kCfMaxKeys distinct keys takes a stable radix sort of its entries by (C1, C2) instead,
 // whose runs are folded left to right.
 
+// Members per community: each wave takes 64 consecutive vertices, and the first lane of each
+// run of equal ids adds the run's length (one atomic per run instead of per vertex).
 __global__ void k_count_members(const uint32_t* __restrict__ newid, int64_t nv, int64_t* __restrict__ cnt)
 {
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd((unsigned long long*)&cnt[newid[v]], 1ull);
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < nv; base += stride) {
+        const int64_t v = base + lane;
+        const bool valid = v < nv;  // the valid lanes are a prefix of the wave
+        const uint32_t c = valid ? newid[v] : 0xFFFFFFFFu;
+        const uint32_t before = __shfl_up(c, 1);
+        const bool head = valid && (lane == 0 || before != c);
+        const unsigned long long heads = __ballot(head);
+        const int nvalid = __popcll(__ballot(valid));
+        if (head) {
+            const unsigned long long later = lane == 63 ? 0ull : heads >> (lane + 1);
+            const int end = later ? lane + 1 + __builtin_ctzll(later) : nvalid;
+            atomicAdd((unsigned long long*)&cnt[c], (unsigned long long)(end - lane));
+        }
+    }
 }
 
 __global__ void k_iota_u32(uint32_t* __restrict__ p, int64_t n)
@@ -342,17 +361,24 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit(
     double* __restrict__ vals)
 {
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
-        if (xcnt[c] == 0 || xcnt[c] > kOvfBig) continue;  // k_ovf_emit_big takes the giant ones
-        const MemberRows mr(cstart, moff, mrow, c, lane);
-        const int64_t o = xoff[c] - mr.s0;
-        for (int64_t base = mr.s0; base < mr.s1; base += 64) {
-            const int64_t p = base + lane;
-            const int64_t k = mr.csr_index(p, moff, mrow);
-            if (p < mr.s1) {
-                const DEdge e = edges[k];
-                keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
-                vals[o + p] = e.weight;
+    // each wave takes windows of 64 communities: one coalesced read of their entry counts
+    // finds the few that overflowed (k_ovf_emit_big takes the giant ones)
+    const int64_t nwin = (nc + 63) / 64;
+    for (int64_t win = (int64_t)blockIdx.x * kCfWaves + w; win < nwin; win += (int64_t)gridDim.x * kCfWaves) {
+        const int64_t cl = win * 64 + lane;
+        const int64_t x = cl < nc ? xcnt[cl] : 0;
+        for (unsigned long long todo = __ballot(x != 0 && x <= kOvfBig); todo; todo &= todo - 1) {
+            const int64_t c = win * 64 + __builtin_ctzll(todo);
+            const MemberRows mr(cstart, moff, mrow, c, lane);
+            const int64_t o = xoff[c] - mr.s0;
+            for (int64_t base = mr.s0; base < mr.s1; base += 64) {
+                const int64_t p = base + lane;
+                const int64_t k = mr.csr_index(p, moff, mrow);
+                if (p < mr.s1) {
+                    const DEdge e = edges[k];
+                    keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
+                    vals[o + p] = e.weight;
+                }
             }
         }
     }
@@ -457,10 +483,13 @@ __global__ void __launch_bounds__(64 * kCfWaves) k_row_copy(int64_t nc, const in
                                                            const uint32_t* __restrict__ rkey,
                                                            const double* __restrict__ rval, DEdge* __restrict__ out)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int64_t c = (int64_t)blockIdx.x * kCfWaves + w; c < nc; c += (int64_t)gridDim.x * kCfWaves) {
+    // 16 lanes per community (coarse rows average a few dozen entries)
+    constexpr int kLanes = 16;
+    const int sub = threadIdx.x % kLanes;
+    const int64_t groups = (int64_t)gridDim.x * blockDim.x / kLanes;
+    for (int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanes; c < nc; c += groups) {
         const int64_t s0 = moff[cstart[c]], len = rowlen[c], o = out_offs[c];
-        for (int64_t i = lane; i < len; i += 64) {
+        for (int64_t i = sub; i < len; i += kLanes) {
             DEdge e;
             e.tail = (int64_t)rkey[s0 + i];
             e.weight = rval[s0 + i];
@@ -486,7 +515,7 @@ static void overflow_rows(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid,
     const size_t x1 = (size_t)std::max<int64_t>(nx, 1), c1n = (size_t)nc + 1;
     DBuf<unsigned long long> k0(ctx, x1), k1(ctx, x1);
     DBuf<double> v0(ctx, x1), v1(ctx, x1);
-    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, moff, mrow,
+    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for((nc + 63) / 64, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, moff, mrow,
               (const DEdge*)g->edges, newid, xcnt, xoff, bits, k0.p, v0.p);
     if (nx > kOvfBig) {  // a community that big may exist
         DBuf<unsigned long long> nbig(ctx, 1);
