@@ -713,10 +713,17 @@ def main():
     }
 
     if not args.no_ttq:
-        t0 = time.perf_counter()
-        r = dg.run(threshold_cycling=True)
-        ttq = time.perf_counter() - t0
-        out["time_to_final_q"] = {"seconds": ttq, "final_q": r.final_q, "levels": len(r.level_q),
+        # two whole runs on the same graph (the first also makes the context's allocations
+        # for every level); the faster one is reported, both are listed
+        runs = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = dg.run(threshold_cycling=True)
+            runs.append((time.perf_counter() - t0, r))
+        ttq, r = min(runs, key=lambda x: x[0])
+        same = all(x.call_q == r.call_q and x.final_q == r.final_q for _, x in runs)
+        out["time_to_final_q"] = {"seconds": ttq, "runs_seconds": [t for t, _ in runs], "runs_identical": same,
+                                  "final_q": r.final_q, "levels": len(r.level_q),
                                   "calls": len(r.call_q), "total_iters": r.total_iters,
                                   "clustering_seconds": r.clustering_seconds, "flags": "-i"}
         ref = reference_run("rgg", args.seed, args.n, args.extra, ["-i"])

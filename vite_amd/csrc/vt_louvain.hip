@@ -15,10 +15,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <initializer_list>
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -2655,11 +2657,11 @@ int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t)
     });
 }
 
-// Coordinate descent over the community-record layout, the bucket bounds (one chunk or up to
-// two per row at each group size), the per-bucket block cap (then the bounds again when the
+// Coordinate descent over the community-record layout, two whole bound profiles, the bucket
+// bounds (one chunk or up to two per row at each group size), the per-bucket block cap (then the bounds again when the
 // cap changed) and the wide-row table sizes,
 // each candidate timed with HIP events over `reps` sweeps of the same state; a candidate
-// must win by 1% to replace the incumbent.
+// must win by 1%, timed twice against the incumbent timed twice, to replace it.
 int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning* out)
 {
     if (!ctx || !g || prep < 0 || reps < 1) return VT_ERR_ARG;
@@ -2692,18 +2694,61 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
             };
             vt_tuning best = start;
             double tb = time_it(best);
+            // a candidate that times 1% faster is timed again together with the incumbent,
+            // and replaces it only if its mean still wins (one timing can win by noise)
+            // VITE_AMD_AUTOTUNE_LOG=1 prints every candidate's timing to stderr
+            const bool log = std::getenv("VITE_AMD_AUTOTUNE_LOG") != nullptr;
+            auto show = [&](const char* what, const vt_tuning& t, double ms) {
+                if (!log) return;
+                std::fprintf(stderr, "autotune %s: bounds {%d,%d,%d,%d,%d,%d} blocks %d wide {%d,%d} records %d: %.4f ms\n",
+                             what, t.bucket_max_degree[0], t.bucket_max_degree[1], t.bucket_max_degree[2],
+                             t.bucket_max_degree[3], t.bucket_max_degree[4], t.bucket_max_degree[5],
+                             t.blocks_per_bucket, t.wide_table_max_degree[0], t.wide_table_max_degree[1],
+                             t.vertex_records, ms);
+            };
+            show("start", best, tb);
+            auto consider = [&](const vt_tuning& t) {
+                const double tt = time_it(t);
+                show("candidate", t, tt);
+                if (tt >= 0.99 * tb) return;
+                const double mt = 0.5 * (tt + time_it(t));
+                const double mb = 0.5 * (tb + time_it(best));
+                if (mt < 0.99 * mb) {
+                    best = t;
+                    tb = mt;
+                    show("accepted", t, mt);
+                } else {
+                    tb = mb;
+                }
+            };
             // community records against separate arrays: first, since it changes which
             // bucket bounds and caps are best, and again at the end
             auto try_records = [&]() {
                 vt_tuning t = best;
                 t.vertex_records = 1 - best.vertex_records;
-                const double tt = time_it(t);
-                if (tt < 0.99 * tb) {
-                    best = t;
-                    tb = tt;
-                }
+                consider(t);
             };
             try_records();
+            // whole bound profiles first, which single-bucket moves reach only through
+            // intermediate settings that are slower on their own: the widest group that a
+            // row fills in one chunk (random graphs, where every row's reads are one round
+            // trip whatever the group), and one chunk per row at each group size
+            // each with and without the community records: on a skewed graph the two gain
+            // only together (BA n = 2^22: 2.29 ms against 2.7 for either alone)
+            auto try_profiles = [&]() {
+                const vt_tuning before = best;
+                for (const std::array<int32_t, 6>& prof : {std::array<int32_t, 6>{7, 7, 15, 31, 191, 1023},
+                                                            std::array<int32_t, 6>{3, 7, 15, 31, 63, 1023}}) {
+                    for (int rec = 0; rec < 2; rec++) {
+                        vt_tuning t = best;
+                        for (int b = 0; b < 6; b++) t.bucket_max_degree[b] = prof[b];
+                        t.vertex_records = rec;
+                        consider(t);
+                    }
+                }
+                return std::memcmp(before.bucket_max_degree, best.bucket_max_degree, sizeof(before.bucket_max_degree)) != 0;
+            };
+            try_profiles();
             auto try_bounds = [&]() {
                 for (int b = 0; b < 5; b++) {
                     const int G = 4 << b;
@@ -2713,28 +2758,26 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
                         if (cand == best.bucket_max_degree[b]) continue;
                         vt_tuning t = best;
                         t.bucket_max_degree[b] = cand;
-                        const double tt = time_it(t);
-                        if (tt < 0.99 * tb) {
-                            best = t;
-                            tb = tt;
-                        }
+                        consider(t);
                     }
                 }
             };
             try_bounds();
-            const vt_tuning after_bounds = best;
-            for (int cap : {2048, 4096, 8192, 16384, 65536}) {
-                if (cap == best.blocks_per_bucket) continue;
-                vt_tuning t = best;
-                t.blocks_per_bucket = cap;
-                const double tt = time_it(t);
-                if (tt < 0.99 * tb) {
-                    best = t;
-                    tb = tt;
+            auto try_caps = [&]() {
+                const int32_t before = best.blocks_per_bucket;
+                for (int cap : {2048, 4096, 8192, 16384, 65536}) {
+                    if (cap == best.blocks_per_bucket) continue;
+                    vt_tuning t = best;
+                    t.blocks_per_bucket = cap;
+                    consider(t);
                 }
+                return best.blocks_per_bucket != before;
+            };
+            // a new cap moves the best bounds: the profiles and one more pass over the bounds
+            if (try_caps()) {
+                try_profiles();
+                try_bounds();
             }
-            // a new cap moves the best bounds: one more pass over them
-            if (best.blocks_per_bucket != after_bounds.blocks_per_bucket) try_bounds();
             // wide-row table sizes: fuller small tables (more waves per CU) against emptier
             // large ones (shorter probes), when the graph has wide rows at all
             const ColorSweep& cs0 = st->ph.sweeps[0];
@@ -2747,14 +2790,13 @@ int vt_autotune(vt_ctx* ctx, vt_graph* g, int32_t prep, int32_t reps, vt_tuning*
                         vt_tuning t = best;
                         t.wide_table_max_degree[k] = cand;
                         if (t.wide_table_max_degree[0] > t.wide_table_max_degree[1]) continue;
-                        const double tt = time_it(t);
-                        if (tt < 0.99 * tb) {
-                            best = t;
-                            tb = tt;
-                        }
+                        consider(t);
                     }
                 }
             }
+            // the profiles once more against the final choice (the cap and tables they are
+            // judged with; a profile that wins can want another cap), then the record layout
+            if (try_profiles()) try_caps();
             try_records();
             ctx->profiling = prof;
             ctx->tuning = best;

@@ -195,9 +195,9 @@ struct MemberRows {
 };
 
 constexpr int kCfWaves = 4;
-// The fold's blocks are one wave each: a block holds its LDS tables until its last wave
-// ends, and the waves of a block take communities of very different sizes, so 4-wave blocks
-// left about half of the resident-wave slots the LDS allows idle.
+// The fold's (and the overflow emission's) blocks are one wave each: a block holds its
+// resources until its last wave ends, and the waves of a block take communities of very
+// different sizes, so 4-wave blocks left about half of the resident-wave slots idle.
 constexpr int kFoldWaves = 1;
 constexpr int kCfLog = 8;
 constexpr int kCfCap = 1 << kCfLog;       // table slots per wave
@@ -354,31 +354,24 @@ __global__ void k_ovf_big_list(int64_t nc, const int64_t* __restrict__ xcnt, uns
         if (xcnt[c] > kOvfBig) list[atomicAdd(count, 1ull)] = c;
 }
 
-__global__ void __launch_bounds__(64 * kCfWaves) k_ovf_emit(
+__global__ void __launch_bounds__(64 * kFoldWaves) k_ovf_emit(
     int64_t nc, const int64_t* __restrict__ cstart, const int64_t* __restrict__ moff, const int64_t* __restrict__ mrow,
     const DEdge* __restrict__ edges, const uint32_t* __restrict__ newid,
     const int64_t* __restrict__ xcnt, const int64_t* __restrict__ xoff, int bits, unsigned long long* __restrict__ keys,
     double* __restrict__ vals)
 {
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // each wave takes windows of 64 communities: one coalesced read of their entry counts
-    // finds the few that overflowed (k_ovf_emit_big takes the giant ones)
-    const int64_t nwin = (nc + 63) / 64;
-    for (int64_t win = (int64_t)blockIdx.x * kCfWaves + w; win < nwin; win += (int64_t)gridDim.x * kCfWaves) {
-        const int64_t cl = win * 64 + lane;
-        const int64_t x = cl < nc ? xcnt[cl] : 0;
-        for (unsigned long long todo = __ballot(x != 0 && x <= kOvfBig); todo; todo &= todo - 1) {
-            const int64_t c = win * 64 + __builtin_ctzll(todo);
-            const MemberRows mr(cstart, moff, mrow, c, lane);
-            const int64_t o = xoff[c] - mr.s0;
-            for (int64_t base = mr.s0; base < mr.s1; base += 64) {
-                const int64_t p = base + lane;
-                const int64_t k = mr.csr_index(p, moff, mrow);
-                if (p < mr.s1) {
-                    const DEdge e = edges[k];
-                    keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
-                    vals[o + p] = e.weight;
-                }
+    for (int64_t c = (int64_t)blockIdx.x * kFoldWaves + w; c < nc; c += (int64_t)gridDim.x * kFoldWaves) {
+        if (xcnt[c] == 0 || xcnt[c] > kOvfBig) continue;  // k_ovf_emit_big takes the giant ones
+        const MemberRows mr(cstart, moff, mrow, c, lane);
+        const int64_t o = xoff[c] - mr.s0;
+        for (int64_t base = mr.s0; base < mr.s1; base += 64) {
+            const int64_t p = base + lane;
+            const int64_t k = mr.csr_index(p, moff, mrow);
+            if (p < mr.s1) {
+                const DEdge e = edges[k];
+                keys[o + p] = ((unsigned long long)c << bits) | newid[e.tail];
+                vals[o + p] = e.weight;
             }
         }
     }
@@ -515,7 +508,7 @@ static void overflow_rows(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid,
     const size_t x1 = (size_t)std::max<int64_t>(nx, 1), c1n = (size_t)nc + 1;
     DBuf<unsigned long long> k0(ctx, x1), k1(ctx, x1);
     DBuf<double> v0(ctx, x1), v1(ctx, x1);
-    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for((nc + 63) / 64, kCfWaves, 65536), 64 * kCfWaves, 0, nc, cstart, moff, mrow,
+    VT_LAUNCH(ctx, "rb_ovf_emit", k_ovf_emit, grid_for(nc, kFoldWaves, 262144), 64 * kFoldWaves, 0, nc, cstart, moff, mrow,
               (const DEdge*)g->edges, newid, xcnt, xoff, bits, k0.p, v0.p);
     if (nx > kOvfBig) {  // a community that big may exist
         DBuf<unsigned long long> nbig(ctx, 1);
