@@ -1,6 +1,5 @@
 // Context, memory, timing, graph transfer and device primitives for vite_amd.
 #include <hip/hip_runtime.h>
-#include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -148,17 +147,44 @@ void flush_timings(vt_ctx* ctx)
 void sync(vt_ctx* ctx)
 {
     VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stage_off = 0;
     flush_timings(ctx);
 }
 
+// Copies up to kStageMax bytes go through the context's pinned staging ring (kStageCap
+// bytes): the source is copied there at once, so the caller may reuse it on return, and the
+// device copy is queued without waiting for the stream.  Larger copies (graph uploads) copy
+// from the caller's memory directly.
+constexpr size_t kStageCap = size_t(16) << 20;
+constexpr size_t kStageMax = size_t(4) << 20;
+
 void h2d(vt_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
-    if (bytes) VT_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (!bytes) return;
+    if (bytes > kStageMax) {
+        VT_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return;
+    }
+    if (!ctx->stage) {
+        VT_HIP(ctx, hipHostMalloc((void**)&ctx->stage, kStageCap, hipHostMallocDefault));
+        ctx->stage_off = 0;
+    }
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (ctx->stage_off + need > kStageCap) {
+        // the ring is full: its earlier copies must have run before it is overwritten
+        VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->stage_off = 0;
+    }
+    unsigned char* slot = ctx->stage + ctx->stage_off;
+    std::memcpy(slot, src, bytes);
+    VT_HIP(ctx, hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream));
+    ctx->stage_off += need;
 }
 void d2h(vt_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
     if (bytes) VT_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stage_off = 0;  // the stream has drained: every staged copy has run
 }
 void d2d(vt_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
@@ -648,6 +674,7 @@ int vt_ctx_destroy(vt_ctx* ctx)
     for (auto& p : ctx->pending) { ctx->event_pool.push_back(p.e0); ctx->event_pool.push_back(p.e1); }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
     delete ctx;
     return VT_OK;
 }

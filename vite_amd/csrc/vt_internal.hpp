@@ -58,8 +58,11 @@ struct vt_ctx {
     // per-iteration trace of the last phase
     std::vector<double> trace_q;
     std::vector<int64_t> trace_moves;
-    // pinned host staging for small readbacks
-    void* pinned = nullptr;
+    // pinned host staging for small host-to-device copies: a copy from pageable memory
+    // waits for the stream's earlier work; a copy from here is queued like a kernel.  The
+    // ring restarts whenever the stream has drained (sync, d2h).
+    unsigned char* stage = nullptr;
+    size_t stage_off = 0;
     // sweep launch shape (performance only: every setting gives the same results)
     vt_tuning tuning = {{7, 15, 31, 63, 127, 1023}, 16384};  // replaced by default_tuning() at creation
 };
