@@ -2,7 +2,7 @@
 device memory (vt_dev_scan, vt_dev_sort_pairs): checked against numpy's exclusive cumsum
 and stable argsort on the same inputs.  Values carry each pair's input position, so equal
 keys must come out in input order.  Counts below the grids' capacity exercise the blocks
-that exit early; tile boundaries (2048 pairs), both digit widths (8 bits up to 24 key bits, 9 bits
+that exit early; tile boundaries (4096 pairs), both digit widths (8 bits up to 24 key bits, 9 bits
 above) and every pass count are covered."""
 import ctypes as C
 
@@ -26,7 +26,7 @@ def _sort(ctx, keys, vals, bits, capacity):
     return ko, vo
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 2047, 2048, 2049, 4097, 100_003, 3_000_000])
+@pytest.mark.parametrize("n", [0, 1, 2, 2047, 2048, 2049, 4095, 4096, 4097, 100_003, 3_000_000])
 @pytest.mark.parametrize("bits", [8, 17, 24, 25, 26, 27, 32])
 def test_dev_sort_is_stable_sort(ctx, n, bits):
     rng = np.random.default_rng(n * 131 + bits)

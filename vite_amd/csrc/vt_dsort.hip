@@ -7,7 +7,7 @@
 // at once.  They record no memset, allocate nothing once their scratch exists, and make no
 // host read, so a sequence of them can be captured and replayed.
 //
-// Sort: least-significant-digit radix sort, 8- or 9-bit digits, tiles of 2048 pairs, each
+// Sort: least-significant-digit radix sort, 8- or 9-bit digits, tiles of 4096 pairs, each
 // block sorting a contiguous run of tiles.  Per pass a histogram kernel counts each block's
 // digits (digit-major table: digit d of block b at d * blocks + b), one block per digit scans
 // its row and totals it, and a scatter kernel places each pair at its digit's global offset
@@ -30,8 +30,13 @@ constexpr int64_t kScanChunk = (int64_t)kScanBlock * kScanItems;  // 2048 values
 constexpr int kPartBlock = 1024;
 
 constexpr int kSortBlock = 256;
-constexpr int kSortRounds = 8;
-constexpr int64_t kTile = (int64_t)kSortBlock * kSortRounds;  // 2048 pairs per tile
+// 4096-pair tiles: on 18 M pairs with 24-bit keys a sort takes 428 us against 472 us with
+// 2048-pair tiles (longer runs per digit in each tile's writes) and 500 us with 6144
+#ifndef VT_SORT_ROUNDS
+#define VT_SORT_ROUNDS 16
+#endif
+constexpr int kSortRounds = VT_SORT_ROUNDS;
+constexpr int64_t kTile = (int64_t)kSortBlock * kSortRounds;  // pairs per tile
 constexpr int kWaves = kSortBlock / 64;
 constexpr int kMaxDigitBits = 9;                // digits of 8 or 9 bits
 constexpr int kMaxBins = 1 << kMaxDigitBits;    // bins of a 9-bit digit
@@ -199,7 +204,7 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_offsets(const int64_t* __r
     if (threadIdx.x == 0) dtot[d] = (int32_t)total;
 }
 
-// Each block places its run of tiles in order.  Per tile: every wave ranks its 512 pairs (8
+// Each block places its run of tiles in order.  Per tile: every wave ranks its 1024 pairs (16
 // steps of 64 lanes, in input order) among the wave's pairs of the same digit, from ballots
 // over the digit bits and a per-wave running count in LDS; the waves' counts give each
 // digit's place in the tile (a scan over digits) and each wave's place inside the digit; the
@@ -216,7 +221,7 @@ __global__ void __launch_bounds__(kSortBlock) k_dsort_scatter(const uint32_t* __
                                                             uint32_t* __restrict__ keys_out,
                                                             uint32_t* __restrict__ vals_out)
 {
-    constexpr int kSteps = kSortRounds;                 // 8 steps of 64 pairs per wave and tile
+    constexpr int kSteps = kSortRounds;                 // steps of 64 pairs per wave and tile
     __shared__ int32_t s_wrun[kWaves][kMaxBins];        // per wave: running count, then place
     __shared__ int32_t s_outt[kMaxBins];                // output position of the tile's staged index 0
     __shared__ int64_t s_wave[kWaves];
