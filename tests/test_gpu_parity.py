@@ -193,15 +193,18 @@ def test_rebuild(ctx, name):
 
 
 @pytest.mark.parametrize("name", graphs.ALL)
-@pytest.mark.parametrize("part", ["identity", "random", "one"])
+@pytest.mark.parametrize("part", ["identity", "random", "one", "blocks64", "random200"])
 def test_rebuild_partitions(ctx, name, part):
     """Partitions no phase produces: singletons (every fine row becomes a coarse row; the hub
     has more distinct neighbours than the per-community table holds, so it takes the sort
-    path), random communities, and one community holding every vertex."""
+    path), random communities, one community holding every vertex, blocks of 64 consecutive
+    vertices (long concatenated rows, few neighbours), and 200 random communities (on most
+    graphs, communities with more distinct neighbours than the fold's table holds)."""
     g, dg = _upload(ctx, name)
     rng = np.random.default_rng(7)
     cv = {"identity": np.arange(g.nv), "random": rng.integers(0, max(1, g.nv // 3), g.nv),
-          "one": np.zeros(g.nv)}[part].astype(np.int64)
+          "one": np.zeros(g.nv), "blocks64": np.arange(g.nv) // 64,
+          "random200": rng.integers(0, min(200, g.nv), g.nv)}[part].astype(np.int64)
     r_o = po.rebuild(g, cv)
     r = dg.rebuild(cv).download()
     assert np.array_equal(r.offs, r_o.offs)
