@@ -5,7 +5,8 @@
 // the events cannot take its size from the host.  These kernels read the count from device
 // memory, size their grids for a host-known capacity and let the blocks past the count exit
 // at once.  They record no memset, allocate nothing once their scratch exists, and make no
-// host read, so a sequence of them can be captured and replayed.
+// host read, so a sequence of them can be captured and replayed.  A single graph's
+// iterations use them too, so that an iteration reads nothing back before its modularity.
 //
 // Sort: least-significant-digit radix sort, 8- or 9-bit digits, tiles of 4096 pairs, each
 // block sorting a contiguous run of tiles.  Per pass a histogram kernel counts each block's
