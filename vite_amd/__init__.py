@@ -447,11 +447,14 @@ class Batch:
         n, nv, ne = C.c_int32(), C.c_int64(), C.c_int64()
         ctx.check(_lib.vt_batch_info(self.h, C.byref(n), C.byref(nv), C.byref(ne)))
         self.total_nv, self.total_ne = nv.value, ne.value
+        # the per-graph results (about 10 KB each), allocated once: vt_batch_run clears and
+        # fills every entry on each run
+        self._res = (_RunResult * self.n)()
 
     def run(self, use_graph=True, **kw):
         """Returns (list of RunResult in graph order, stats dict)."""
         rp = _run_params(flags=0 if use_graph else VT_FLAG_NO_GRAPH, **kw)
-        res = (_RunResult * self.n)()
+        res = self._res
         st = _BatchStats()
         self.ctx.check(_lib.vt_batch_run(self.ctx.h, self.h, C.byref(rp), res, C.byref(st)))
         return [_run_result(r) for r in res], dict(phases=st.phases, rounds=st.rounds, seconds=st.seconds,
