@@ -367,7 +367,8 @@ def batched_form(ctx, args):
     for g in graphs:
         g.free()
     ctx.set_tuning(**saved)
-    return {"graphs": count, "graphs_per_s": count / batch_s, "seconds": batch_s, "tuning": tuning,
+    return {"graphs": count, "graphs_per_s": count / batch_s, "seconds": batch_s,
+            "library_seconds": st["seconds"], "library_graphs_per_s": count / st["seconds"], "tuning": tuning,
             "one_per_launch_graphs_per_s": plain_gps, "one_per_launch_sample": len(sample),
             "speedup": (count / batch_s) / plain_gps, "identical_per_graph_results": same,
             "phases": st["phases"], "rounds": st["rounds"], "total_nv": b.total_nv, "total_ne": b.total_ne,
