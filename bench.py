@@ -389,7 +389,11 @@ def skewed_form(ctx, args):
     log(f"skewed: BA n={dg.nv} m=8, {dg.ne} entries, max degree {int(deg.max())} (generated in {gen_s:.2f}s)")
     saved = ctx.tuning
     tuning = saved
-    if not args.no_autotune:
+    if args.skewed_tuning:
+        ctx.set_tuning(**args.skewed_tuning)
+        tuning = ctx.tuning
+        log(f"skewed tuning as given: {tuning}")
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tuning = ctx.autotune(dg, prep=args.prep, reps=3)
         log(f"skewed autotune: {tuning} in {time.perf_counter() - t0:.2f}s")
@@ -610,6 +614,9 @@ def main():
     ap.add_argument("--batch-sample", type=int, default=256,
                     help="graphs of the batch also run one graph per launch (the comparison)")
     ap.add_argument("--skewed-n", type=int, default=1 << 22, help="BA vertices of the skewed-degree run (0 = skip)")
+    ap.add_argument("--skewed-tuning", type=json.loads, default=None, metavar="JSON",
+                    help="launch tuning of the skewed run, as the bench line's skewed.tuning object, instead of "
+                         "the autotuner's (profiling passes then all run the same kernels)")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.9,

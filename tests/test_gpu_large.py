@@ -84,6 +84,40 @@ def test_ba_hubs_color_et_run_exact(ctx, ba_hubs):
     assert r.level_ne == r_o.level_ne
 
 
+@pytest.mark.timeout(300)
+def test_ba_2p22_hub_key_widths(ctx):
+    """The hub path's two key widths on the skewed-degree graph (BA n = 2^22, m = 8), whose
+    keys are (hub index, neighbour community): with the default bounds its few hundred rows
+    of degree 1024+ are the hubs and the keys fit 32 bits (22 community bits, 9 hub bits), so
+    the device sort runs; with every row above degree 130 sent to the hub path, some 17 K
+    hubs need 64-bit keys and the library sort runs.  Both sweeps, from the state two phase-0
+    iterations leave, equal the oracle's bit for bit."""
+    dg = ctx.gen_ba(2027, 1 << 22, 8)
+    g = _host(dg)
+    default = ctx.tuning
+    try:
+        for width, bounds in ((32, default["bucket_max_degree"]), (64, [7, 15, 47, 63, 127, 130])):
+            ctx.set_tuning(bucket_max_degree=bounds)
+            ctx.set_profiling(True)
+            ctx.reset_stats()
+            st = dg.phase_begin(thresh=1e-3)
+            for _ in range(2):
+                st.step()
+            st.sweep()
+            s = st.state()
+            st.end(want_cvect=False)
+            launches = {w: ctx.kernel_stats(f"hub_keys{w}")[1] for w in (32, 64)}
+            ctx.set_profiling(False)
+            assert launches[width] > 0 and launches[96 - width] == 0, launches
+            t_o, cw_o, _, _ = po.sweep(g, s["curr"], s["csize"], s["cdeg"], s["vdeg"], s["constant"])
+            assert np.array_equal(s["target"], t_o), width
+            assert np.array_equal(s["cw"].view(np.uint64), cw_o.view(np.uint64)), width
+    finally:
+        ctx.set_profiling(False)
+        ctx.set_tuning(**default)
+    dg.free()
+
+
 def test_ba_giant_community_of_few_hubs_rebuild(ctx):
     """A community of at most 64 members holding more than 2^16 entries: the rebuild's
     overflow path emits it with every wave (k_ovf_emit_big), each lane finding its member by

@@ -1134,22 +1134,25 @@ __global__ void __launch_bounds__(64) k_move_hbm(MoveArgs a, const int32_t* __re
 
 // Keys (hub j << cbits | community of the tail; communities < 2^cbits) and values (edge
 // index) for hub j's row.
+// Hub keys are (hub index << cbits | neighbour community): 32-bit when they fit, else 64-bit.
+template <class K>
 __global__ void k_hub_keys(const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
                            const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
-                           const int64_t* __restrict__ curr, int cbits, unsigned long long* __restrict__ keys,
+                           const int64_t* __restrict__ curr, int cbits, K* __restrict__ keys,
                            uint32_t* __restrict__ vals)
 {
     for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
         const int64_t q0 = hoffs[j], n = hoffs[j + 1] - q0, e0 = offs[hubs[j]];
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
             const int64_t k = e0 + i;
-            keys[q0 + i] = ((unsigned long long)j << cbits) | (uint32_t)curr[edges[k].tail];
+            keys[q0 + i] = ((K)j << cbits) | (K)(uint32_t)curr[edges[k].tail];
             vals[q0 + i] = (uint32_t)k;
         }
     }
 }
 
-__global__ void k_hub_heads(const unsigned long long* __restrict__ keys, int64_t n, int32_t* __restrict__ head)
+template <class K>
+__global__ void k_hub_heads(const K* __restrict__ keys, int64_t n, int32_t* __restrict__ head)
 {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         head[q] = (q == 0 || keys[q] != keys[q - 1]) ? 1 : 0;
@@ -1200,9 +1203,10 @@ __device__ __forceinline__ double hub_run_fold(const DEdge* __restrict__ edges, 
 
 constexpr int kHubBlock = 256;
 
+template <class K>
 __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_t* __restrict__ hubs, int32_t nh,
                                                         const int64_t* __restrict__ hoffs,
-                                                        const unsigned long long* __restrict__ keys,
+                                                        const K* __restrict__ keys,
                                                         const uint32_t* __restrict__ vals,
                                                         const int32_t* __restrict__ pos,
                                                         const int32_t* __restrict__ starts, int cbits)
@@ -1213,7 +1217,7 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
     unsigned inactive = 0;
     const double constant = *a.constant;
     const int64_t total = hoffs[nh];
-    const int32_t nruns = total ? pos[total - 1] + (int32_t)(keys[total - 1] != (total > 1 ? keys[total - 2] : ~0ull)) : 0;
+    const int32_t nruns = total ? pos[total - 1] + (int32_t)(total == 1 || keys[total - 1] != keys[total - 2]) : 0;
     for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
         const int32_t v = hubs[j];
         if (a.active && !a.active[v]) {
@@ -1226,7 +1230,7 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
         const int32_t r0 = hoffs[j] < total ? pos[hoffs[j]] : nruns;
         const int32_t r1 = hoffs[j + 1] < total ? pos[hoffs[j + 1]] : nruns;
         const int64_t cc = a.curr[v];
-        const unsigned long long ckey = ((unsigned long long)j << cbits) | (uint32_t)cc;
+        const K ckey = ((K)j << cbits) | (K)(uint32_t)cc;
         if (threadIdx.x == 0) {
             // the own community's run, if any neighbour is in it
             int32_t lo = r0, hi = r1;
@@ -2078,6 +2082,45 @@ struct Phase {
         return a;
     }
 
+    // Hub rows: a key per hub edge, (hub index << cbits | neighbour community) with the edge
+    // index as the value, one stable sort of all of them, the run bounds, then a workgroup per
+    // hub.  Keys of at most 32 bits (the usual case: community ids below 2^22 and a few hundred
+    // hubs) take the device sort and scan, counted by hoffs[nh] (the hub entry total); wider
+    // keys take the 64-bit library sort.
+    template <class K>
+    void hub_sweep(const MoveArgs& a, const ColorSweep& cs, int64_t total, int cbits, int kbits)
+    {
+        const int32_t nh = (int32_t)cs.hub_n;
+        const int32_t* hubs = (const int32_t*)(lists.p + cs.hub_off);
+        const int64_t* hoffs = hub_offs[cs.index].p;
+        const size_t t1 = (size_t)std::max<int64_t>(total, 1);
+        K* k0 = (K*)scratch(ctx, "hs_k0", 8 * t1);
+        K* k1 = (K*)scratch(ctx, "hs_k1", 8 * t1);
+        uint32_t* v0 = (uint32_t*)scratch(ctx, "hs_v0", 4 * t1);
+        uint32_t* v1 = (uint32_t*)scratch(ctx, "hs_v1", 4 * t1);
+        int32_t* head = (int32_t*)scratch(ctx, "hs_head", 4 * t1);
+        int32_t* pos = (int32_t*)scratch(ctx, "hs_pos", 4 * t1);
+        int32_t* starts = (int32_t*)scratch(ctx, "hs_starts", 4 * t1 + 4);
+        VT_LAUNCH(ctx, sizeof(K) == 4 ? "hub_keys32" : "hub_keys64", k_hub_keys<K>, (unsigned)std::min<int64_t>(nh, 16384),
+                  256, 0, hubs, nh, hoffs,
+                  (const int64_t*)g->offs, (const DEdge*)g->edges, (const int64_t*)curr.p, cbits, k0, v0);
+        bool alt = false;
+        if constexpr (sizeof(K) == 4) {
+            alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, hoffs + nh, 1, total, kbits);
+        } else {
+            sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, kbits, (uint64_t*)k1, v1, &alt);
+        }
+        const K* ks = alt ? k1 : k0;
+        const uint32_t* vs = alt ? v1 : v0;
+        VT_LAUNCH(ctx, "hub_heads", k_hub_heads<K>, grid_for(total, 256, 16384), 256, 0, ks, total, head);
+        if constexpr (sizeof(K) == 4) dev_scan_i32(ctx, head, pos, hoffs + nh, total, nullptr, "hs_part");
+        else exclusive_scan_i32(ctx, head, pos, total);
+        VT_LAUNCH(ctx, "hub_starts", k_hub_starts, grid_for(total, 256, 16384), 256, 0, (const int32_t*)head,
+                  (const int32_t*)pos, total, starts);
+        VT_LAUNCH(ctx, "local_move_hub", k_hub_move<K>, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs, nh,
+                  hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)starts, cbits);
+    }
+
     void sweep(const ColorSweep& cs)
     {
         prepare_narrow();
@@ -2119,30 +2162,10 @@ struct Phase {
         }
         if (cs.hub_n && hub_sort) {
             const int64_t total = hub_total[cs.index];
-            const int32_t nh = (int32_t)cs.hub_n;
-            const int32_t* hubs = (const int32_t*)(lists.p + cs.hub_off);
-            const int64_t* hoffs = hub_offs[cs.index].p;
-            const size_t t1 = (size_t)std::max<int64_t>(total, 1);
-            unsigned long long* k0 = (unsigned long long*)scratch(ctx, "hs_k0", 8 * t1);
-            unsigned long long* k1 = (unsigned long long*)scratch(ctx, "hs_k1", 8 * t1);
-            uint32_t* v0 = (uint32_t*)scratch(ctx, "hs_v0", 4 * t1);
-            uint32_t* v1 = (uint32_t*)scratch(ctx, "hs_v1", 4 * t1);
-            int32_t* head = (int32_t*)scratch(ctx, "hs_head", 4 * t1);
-            int32_t* pos = (int32_t*)scratch(ctx, "hs_pos", 4 * t1);
-            int32_t* starts = (int32_t*)scratch(ctx, "hs_starts", 4 * t1 + 4);
             const int cbits = (int)bits_for(nv > 1 ? nv - 1 : 1);  // community ids < nv
-            VT_LAUNCH(ctx, "hub_keys", k_hub_keys, (unsigned)std::min<int64_t>(nh, 16384), 256, 0, hubs, nh, hoffs,
-                      (const int64_t*)g->offs, (const DEdge*)g->edges, (const int64_t*)curr.p, cbits, k0, v0);
-            bool alt = false;
-            sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, cbits + (int)bits_for(nh), (uint64_t*)k1, v1, &alt);
-            const unsigned long long* ks = alt ? k1 : k0;
-            const uint32_t* vs = alt ? v1 : v0;
-            VT_LAUNCH(ctx, "hub_heads", k_hub_heads, grid_for(total, 256, 16384), 256, 0, ks, total, head);
-            exclusive_scan_i32(ctx, head, pos, total);
-            VT_LAUNCH(ctx, "hub_starts", k_hub_starts, grid_for(total, 256, 16384), 256, 0, (const int32_t*)head,
-                      (const int32_t*)pos, total, starts);
-            VT_LAUNCH(ctx, "local_move_hub", k_hub_move, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs, nh,
-                      hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)starts, cbits);
+            const int kbits = cbits + (int)bits_for(cs.hub_n);
+            if (kbits <= 32 && total < (int64_t(1) << 31)) hub_sweep<uint32_t>(a, cs, total, cbits, kbits);
+            else hub_sweep<unsigned long long>(a, cs, total, cbits, kbits);
         } else if (cs.hub_n) {
             const int blocks = (int)std::min<int64_t>(cs.hub_n, hub_blocks);
             VT_LAUNCH(ctx, "local_move_hub", k_move_hbm, (unsigned)blocks, 64, 0, a, (const int32_t*)(lists.p + cs.hub_off),
