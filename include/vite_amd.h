@@ -71,7 +71,7 @@ int vt_ctx_reset_stats(vt_ctx* ctx);
  * 2 x lanes - 1) or one 64-lane wave with a 2048-slot table (b = 5, at most 1023); longer rows
  * use tables in HBM.  blocks_per_bucket caps the workgroups one bucket gets per launch.
  * Any valid setting gives bit-identical results; only the speed changes.
- * Default {7, 15, 31, 63, 127, 1023}, 16384, {255, 511}, 0. */
+ * Default {7, 15, 31, 63, 127, 1023}, 16384, {255, 511}, 0, 1024. */
 typedef struct {
     int32_t bucket_max_degree[6];
     int32_t blocks_per_bucket;
@@ -85,6 +85,11 @@ typedef struct {
      * with the neighbour's line.  Pays on graphs whose neighbours are scattered (random
      * reads); 0 (the default): separate arrays. */
     int32_t vertex_records;
+    /* Rows above bucket_max_degree[5] (hubs) are grouped by neighbour community with one
+     * sort per sweep; a sweep with at least this many hubs then gives each hub a workgroup,
+     * one with fewer runs every community run in a flat launch (few large hubs would
+     * leave most of the chip idle).  0 <= hub_block_min <= 2^30. */
+    int32_t hub_block_min;
 } vt_tuning;
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out);
 int vt_ctx_set_tuning(vt_ctx* ctx, const vt_tuning* t);

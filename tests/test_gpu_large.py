@@ -85,33 +85,43 @@ def test_ba_hubs_color_et_run_exact(ctx, ba_hubs):
 
 
 @pytest.mark.timeout(300)
-def test_ba_2p22_hub_key_widths(ctx):
-    """The hub path's two key widths on the skewed-degree graph (BA n = 2^22, m = 8), whose
-    keys are (hub index, neighbour community): with the default bounds its few hundred rows
-    of degree 1024+ are the hubs and the keys fit 32 bits (22 community bits, 9 hub bits), so
-    the device sort runs; with every row above degree 130 sent to the hub path, some 17 K
-    hubs need 64-bit keys and the library sort runs.  Both sweeps, from the state two phase-0
-    iterations leave, equal the oracle's bit for bit."""
+def test_ba_2p22_hub_key_widths_and_forms(ctx):
+    """The hub path's two key widths and two move forms on the skewed-degree graph (BA
+    n = 2^22, m = 8), whose keys are (hub index, neighbour community).  With the default
+    bounds its few hundred rows of degree 1024+ are the hubs and the keys fit 32 bits (22
+    community bits, 9 hub bits), so the device sort runs; with every row above degree 130
+    sent to the hub path, some 17 K hubs need 64-bit keys and the library sort runs.  Each
+    width runs with a workgroup per hub and with the flat launches (hub_block_min 0 and
+    2^30).  All four sweeps, from the state two phase-0 iterations leave, equal the oracle's
+    bit for bit."""
     dg = ctx.gen_ba(2027, 1 << 22, 8)
     g = _host(dg)
     default = ctx.tuning
+    oracle = None
     try:
         for width, bounds in ((32, default["bucket_max_degree"]), (64, [7, 15, 47, 63, 127, 130])):
-            ctx.set_tuning(bucket_max_degree=bounds)
-            ctx.set_profiling(True)
-            ctx.reset_stats()
-            st = dg.phase_begin(thresh=1e-3)
-            for _ in range(2):
-                st.step()
-            st.sweep()
-            s = st.state()
-            st.end(want_cvect=False)
-            launches = {w: ctx.kernel_stats(f"hub_keys{w}")[1] for w in (32, 64)}
-            ctx.set_profiling(False)
-            assert launches[width] > 0 and launches[96 - width] == 0, launches
-            t_o, cw_o, _, _ = po.sweep(g, s["curr"], s["csize"], s["cdeg"], s["vdeg"], s["constant"])
-            assert np.array_equal(s["target"], t_o), width
-            assert np.array_equal(s["cw"].view(np.uint64), cw_o.view(np.uint64)), width
+            for blocks in (False, True):
+                ctx.set_tuning(bucket_max_degree=bounds, hub_block_min=0 if blocks else 1 << 30)
+                ctx.set_profiling(True)
+                ctx.reset_stats()
+                st = dg.phase_begin(thresh=1e-3)
+                for _ in range(2):
+                    st.step()
+                st.sweep()
+                s = st.state()
+                st.end(want_cvect=False)
+                n = {k: ctx.kernel_stats(k)[1] for k in ("hub_keys32", "hub_keys64", "hub_move_blocks", "hub_own")}
+                ctx.set_profiling(False)
+                assert n[f"hub_keys{width}"] > 0 and n[f"hub_keys{96 - width}"] == 0, n
+                assert (n["hub_move_blocks"] > 0, n["hub_own"] > 0) == (blocks, not blocks), n
+                if oracle is None:
+                    # every launch shape gives the same state, so one oracle sweep checks all
+                    state = {k: s[k].copy() for k in ("curr", "csize", "cdeg", "vdeg")}
+                    oracle = po.sweep(g, s["curr"], s["csize"], s["cdeg"], s["vdeg"], s["constant"])
+                for k in ("curr", "csize", "cdeg", "vdeg"):
+                    assert np.array_equal(s[k].view(np.uint8), state[k].view(np.uint8)), (width, blocks, k)
+                assert np.array_equal(s["target"], oracle[0]), (width, blocks)
+                assert np.array_equal(s["cw"].view(np.uint64), oracle[1].view(np.uint64)), (width, blocks)
     finally:
         ctx.set_profiling(False)
         ctx.set_tuning(**default)

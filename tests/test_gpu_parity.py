@@ -320,7 +320,11 @@ TUNINGS = [dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=16
            dict(bucket_max_degree=[7, 15, -1, 63, 127, 1023], blocks_per_bucket=512),
            # the group kernels reading the per-vertex community records
            dict(bucket_max_degree=[7, 15, 47, 63, 191, 1023], blocks_per_bucket=16384, vertex_records=1),
-           dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=8, vertex_records=1)]
+           dict(bucket_max_degree=[3, 7, 15, 31, 63, 1023], blocks_per_bucket=8, vertex_records=1),
+           # rows above degree 130 are hubs: a workgroup per hub, then the flat launches, whatever
+           # the hub count
+           dict(bucket_max_degree=[7, 15, 31, 63, 127, 130], blocks_per_bucket=4096, hub_block_min=0),
+           dict(bucket_max_degree=[7, 15, 31, 63, 127, 130], blocks_per_bucket=4096, hub_block_min=1 << 30)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=[str(i) for i in range(len(TUNINGS))])
@@ -374,6 +378,9 @@ def test_tuning_rejects_bounds_beyond_kernel_capacity(ctx):
     for bad in [[384, 511], [255, 768], [511, 300], [-2, 511]]:
         with pytest.raises(vite_amd.VtError):
             ctx.set_tuning(wide_table_max_degree=bad)
+    for bad in [-1, (1 << 30) + 1]:
+        with pytest.raises(vite_amd.VtError):
+            ctx.set_tuning(hub_block_min=bad)
     assert ctx.tuning == default
 
 

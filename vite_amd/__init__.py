@@ -76,12 +76,14 @@ class _RunResult(C.Structure):
 # Every symbol include/vite_amd.h declares, with its ctypes signature.
 class _Tuning(C.Structure):
     _fields_ = [("bucket_max_degree", C.c_int32 * 6), ("blocks_per_bucket", C.c_int32),
-                ("wide_table_max_degree", C.c_int32 * 2), ("vertex_records", C.c_int32)]
+                ("wide_table_max_degree", C.c_int32 * 2), ("vertex_records", C.c_int32),
+                ("hub_block_min", C.c_int32)]
 
 
 def _tuning_dict(t) -> dict:
     return dict(bucket_max_degree=list(t.bucket_max_degree), blocks_per_bucket=t.blocks_per_bucket,
-                wide_table_max_degree=list(t.wide_table_max_degree), vertex_records=t.vertex_records)
+                wide_table_max_degree=list(t.wide_table_max_degree), vertex_records=t.vertex_records,
+                hub_block_min=t.hub_block_min)
 
 
 class _Request(C.Structure):
@@ -241,12 +243,13 @@ class Context:
         return _tuning_dict(t)
 
     def set_tuning(self, bucket_max_degree=None, blocks_per_bucket=None, wide_table_max_degree=None,
-                   vertex_records=None):
+                   vertex_records=None, hub_block_min=None):
         cur = self.tuning
         t = _Tuning((C.c_int32 * 6)(*(bucket_max_degree or cur["bucket_max_degree"])),
                     blocks_per_bucket or cur["blocks_per_bucket"],
                     (C.c_int32 * 2)(*(wide_table_max_degree or cur["wide_table_max_degree"])),
-                    cur["vertex_records"] if vertex_records is None else int(vertex_records))
+                    cur["vertex_records"] if vertex_records is None else int(vertex_records),
+                    cur["hub_block_min"] if hub_block_min is None else int(hub_block_min))
         self.check(_lib.vt_ctx_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, g: "DeviceGraph", prep: int = 2, reps: int = 3) -> dict:

@@ -1135,18 +1135,32 @@ __global__ void __launch_bounds__(64) k_move_hbm(MoveArgs a, const int32_t* __re
 // Keys (hub j << cbits | community of the tail; communities < 2^cbits) and values (edge
 // index) for hub j's row.
 // Hub keys are (hub index << cbits | neighbour community): 32-bit when they fit, else 64-bit.
+// One thread per hub entry q (hub entries are the hubs' rows back to back, hoffs their
+// offsets), the value the entry's CSR index.  Each wave finds the hub of its first entry (the
+// last j with hoffs[j] <= q) by a wave-uniform search; its lanes step from there.
 template <class K>
 __global__ void k_hub_keys(const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
-                           const int64_t* __restrict__ offs, const DEdge* __restrict__ edges,
-                           const int64_t* __restrict__ curr, int cbits, K* __restrict__ keys,
+                           const int64_t* __restrict__ offs, const int32_t* __restrict__ tail32,
+                           const int32_t* __restrict__ curr32, int cbits, K* __restrict__ keys,
                            uint32_t* __restrict__ vals)
 {
-    for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
-        const int64_t q0 = hoffs[j], n = hoffs[j + 1] - q0, e0 = offs[hubs[j]];
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const int64_t k = e0 + i;
-            keys[q0 + i] = ((K)j << cbits) | (K)(uint32_t)curr[edges[k].tail];
-            vals[q0 + i] = (uint32_t)k;
+    const int64_t total = hoffs[nh];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave0 = blockIdx.x * (int64_t)blockDim.x + 64 * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t base = wave0; base < total; base += (int64_t)gridDim.x * blockDim.x) {
+        int32_t lo = 0, hi = nh - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (hoffs[mid] <= base) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t q = base + lane;
+        if (q < total) {
+            int32_t j = lo;
+            while (hoffs[j + 1] <= q) j++;  // hoffs[nh] = total > q
+            const int64_t k = offs[hubs[j]] + (q - hoffs[j]);
+            keys[q] = ((K)j << cbits) | (K)(uint32_t)curr32[tail32[k]];
+            vals[q] = (uint32_t)k;
         }
     }
 }
@@ -1201,14 +1215,137 @@ __device__ __forceinline__ double hub_run_fold(const DEdge* __restrict__ edges, 
     return acc;
 }
 
+// The hub local move as four flat launches, for sweeps with few hubs (a workgroup per hub
+// would leave most of the chip idle: a few hundred hubs, the largest with tens of thousands
+// of runs).  Runs are the sorted hub
+// entries' runs of equal key, starts[] their bounds, pos[] the run index of each entry;
+// own[2j], own[2j+1] = hub j's own-community counter and self-loop weight.
+__device__ __forceinline__ int32_t hub_run_count(const int64_t* __restrict__ hoffs, int32_t nh,
+                                                 const int32_t* __restrict__ pos, const int32_t* __restrict__ head)
+{
+    const int64_t total = hoffs[nh];
+    return total ? pos[total - 1] + head[total - 1] : 0;
+}
+
+// Per hub: its own community's run folded from 0.0 with the self-loops (louvain.cpp:2326-2327,
+// 2440-2441), and the argmax cells cleared.
+template <class K>
+__global__ void k_hub_own(MoveArgs a, const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
+                          const K* __restrict__ keys, const uint32_t* __restrict__ vals, const int32_t* __restrict__ pos,
+                          const int32_t* __restrict__ head, const int32_t* __restrict__ starts, int cbits,
+                          double* __restrict__ own, unsigned long long* __restrict__ best, int32_t* __restrict__ bestc)
+{
+    const int64_t total = hoffs[nh];
+    const int32_t nruns = hub_run_count(hoffs, nh, pos, head);
+    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += gridDim.x * blockDim.x) {
+        best[j] = 0ull;  // the bits of +0.0: only gains > 0 can win
+        bestc[j] = INT_MAX;
+        const int32_t v = hubs[j];
+        if (a.active && !a.active[v]) continue;
+        const int32_t r0 = hoffs[j] < total ? pos[hoffs[j]] : nruns;
+        const int32_t r1 = hoffs[j + 1] < total ? pos[hoffs[j + 1]] : nruns;
+        const K ckey = ((K)j << cbits) | (K)(uint32_t)a.curr[v];
+        int32_t lo = r0, hi = r1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (keys[starts[mid]] < ckey) lo = mid + 1;
+            else hi = mid;
+        }
+        double self = 0.0, ecc = 0.0;
+        if (lo < r1 && keys[starts[lo]] == ckey) ecc = hub_run_fold(a.edges, vals, starts[lo], starts[lo + 1], true, v, self);
+        own[2 * j] = ecc;
+        own[2 * j + 1] = self;
+    }
+}
+
+// Per run of another community: its counter (the run's weights folded left to right, the
+// first as is) and its gain (louvain.cpp:2246-2273); best[j] = the largest positive gain's
+// bits (non-negative doubles order as their bit patterns).
+template <class K>
+__global__ void k_hub_runs(MoveArgs a, const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
+                           const K* __restrict__ keys, const uint32_t* __restrict__ vals, const int32_t* __restrict__ pos,
+                           const int32_t* __restrict__ head, const int32_t* __restrict__ starts, int cbits,
+                           const double* __restrict__ own, double* __restrict__ gains,
+                           unsigned long long* __restrict__ best)
+{
+    const int32_t nruns = hub_run_count(hoffs, nh, pos, head);
+    const double constant = *a.constant;
+    const K cmask = (K)(((unsigned long long)1 << cbits) - 1);
+    const int lane = threadIdx.x & 63;
+    // whole waves step together (the bound is wave-uniform), so the shuffles see every lane
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < nruns;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = (int32_t)(base + lane);
+        const bool valid = r < nruns;
+        int32_t j = -1;
+        double gain = 0.0;
+        if (valid) {
+            const int32_t b = starts[r], e = starts[r + 1];
+            const K key = keys[b];
+            j = (int32_t)(key >> cbits);
+            const int32_t c = (int32_t)(uint32_t)(key & cmask);
+            const int32_t v = hubs[j];
+            if (!(a.active && !a.active[v])) {
+                const int64_t cc = a.curr[v];
+                if ((int64_t)c != cc) {
+                    double unused = 0.0;
+                    const double ey = hub_run_fold(a.edges, vals, b, e, false, v, unused);
+                    const double kv = a.vdeg[v];
+                    const double eix = own[2 * j] - own[2 * j + 1];
+                    const double ax = a.cdeg[cc] - kv;
+                    const double ay = a.cdeg[c];
+                    gain = 2.0 * (ey - eix) - 2.0 * kv * (ay - ax) * constant;
+                }
+            }
+            gains[r] = gain;
+        }
+        // A hub has thousands of runs: when the wave's runs all belong to one hub (lane 0's,
+        // always valid), one atomic carries the wave's largest gain instead of one per run.
+        const unsigned long long gb = gain > 0.0 ? (unsigned long long)__double_as_longlong(gain) : 0ull;
+        const int32_t j0 = __shfl(j, 0);
+        if (__ballot(valid && j != j0) == 0) {
+            unsigned long long m = gb;
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long x = __shfl_xor(m, o);
+                m = x > m ? x : m;
+            }
+            if (lane == 0 && m) atomicMax(&best[j0], m);
+        } else if (gb) {
+            atomicMax(&best[j], gb);
+        }
+    }
+}
+
+// The smallest community among the runs holding their hub's largest gain (the reference's
+// tie-break).
+template <class K>
+__global__ void k_hub_pick(const int32_t* __restrict__ pos, const int32_t* __restrict__ head,
+                           const int64_t* __restrict__ hoffs, int32_t nh, const K* __restrict__ keys,
+                           const int32_t* __restrict__ starts, int cbits, const double* __restrict__ gains,
+                           const unsigned long long* __restrict__ best, int32_t* __restrict__ bestc)
+{
+    const int32_t nruns = hub_run_count(hoffs, nh, pos, head);
+    const K cmask = (K)(((unsigned long long)1 << cbits) - 1);
+    for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nruns; r += gridDim.x * blockDim.x) {
+        const double gain = gains[r];
+        if (!(gain > 0.0)) continue;
+        const K key = keys[starts[r]];
+        const int32_t j = (int32_t)(key >> cbits);
+        if ((unsigned long long)__double_as_longlong(gain) == best[j]) atomicMin(&bestc[j], (int32_t)(uint32_t)(key & cmask));
+    }
+}
+
+// The same move with a workgroup per hub, for sweeps with enough hubs to fill the chip (the
+// coarse levels: thousands of hubs; vt_tuning.hub_block_min sets how many): thread 0 folds the own community's run, the threads take
+// the other runs in turn, and the workgroup reduces their (gain, community) argmax.
 constexpr int kHubBlock = 256;
 
 template <class K>
 __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_t* __restrict__ hubs, int32_t nh,
-                                                        const int64_t* __restrict__ hoffs,
-                                                        const K* __restrict__ keys,
+                                                        const int64_t* __restrict__ hoffs, const K* __restrict__ keys,
                                                         const uint32_t* __restrict__ vals,
                                                         const int32_t* __restrict__ pos,
+                                                        const int32_t* __restrict__ head,
                                                         const int32_t* __restrict__ starts, int cbits)
 {
     __shared__ double s_ecc, s_self;
@@ -1217,7 +1354,8 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
     unsigned inactive = 0;
     const double constant = *a.constant;
     const int64_t total = hoffs[nh];
-    const int32_t nruns = total ? pos[total - 1] + (int32_t)(total == 1 || keys[total - 1] != keys[total - 2]) : 0;
+    const int32_t nruns = hub_run_count(hoffs, nh, pos, head);
+    const K cmask = (K)(((unsigned long long)1 << cbits) - 1);
     for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
         const int32_t v = hubs[j];
         if (a.active && !a.active[v]) {
@@ -1230,9 +1368,8 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
         const int32_t r0 = hoffs[j] < total ? pos[hoffs[j]] : nruns;
         const int32_t r1 = hoffs[j + 1] < total ? pos[hoffs[j + 1]] : nruns;
         const int64_t cc = a.curr[v];
-        const K ckey = ((K)j << cbits) | (K)(uint32_t)cc;
         if (threadIdx.x == 0) {
-            // the own community's run, if any neighbour is in it
+            const K ckey = ((K)j << cbits) | (K)(uint32_t)cc;
             int32_t lo = r0, hi = r1;
             while (lo < hi) {
                 const int32_t mid = (lo + hi) >> 1;
@@ -1240,9 +1377,8 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
                 else hi = mid;
             }
             double self = 0.0, ecc = 0.0;
-            if (lo < r1 && keys[starts[lo]] == ckey) {
+            if (lo < r1 && keys[starts[lo]] == ckey)
                 ecc = hub_run_fold(a.edges, vals, starts[lo], starts[lo + 1], true, v, self);
-            }
             s_ecc = ecc;
             s_self = self;
         }
@@ -1254,8 +1390,8 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
         double bg = 0.0;
         int32_t bc = INT_MAX;
         for (int32_t r = r0 + threadIdx.x; r < r1; r += kHubBlock) {
-            const int64_t b = starts[r], e = starts[r + 1];
-            const int32_t c = (int32_t)(uint32_t)(keys[b] & ((1ull << cbits) - 1));
+            const int32_t b = starts[r], e = starts[r + 1];
+            const int32_t c = (int32_t)(uint32_t)(keys[b] & cmask);
             if ((int64_t)c == cc) continue;
             double unused = 0.0;
             const double ey = hub_run_fold(a.edges, vals, b, e, false, v, unused);
@@ -1283,6 +1419,31 @@ __global__ void __launch_bounds__(kHubBlock) k_hub_move(MoveArgs a, const int32_
         __syncthreads();
     }
     if (threadIdx.x == 0 && inactive && a.inactive_count) atomicAdd(a.inactive_count, (unsigned long long)inactive);
+}
+
+// Per hub: target (with the singleton rule, louvain.cpp:2275-2276) and clusterWeight, or the
+// frozen weight of a hub early termination froze (louvain.cpp:101-104).
+__global__ void k_hub_final(MoveArgs a, const int32_t* __restrict__ hubs, int32_t nh, const double* __restrict__ own,
+                            const unsigned long long* __restrict__ best, const int32_t* __restrict__ bestc)
+{
+    for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nh; j += gridDim.x * blockDim.x) {
+        const int32_t v = hubs[j];
+        if (a.active && !a.active[v]) {
+            a.cw[v] = a.frozen[v];
+            if (a.inactive_count) atomicAdd(a.inactive_count, 1ull);
+            continue;
+        }
+        const int64_t cc = a.curr[v];
+        const double ecc = own[2 * j];
+        int64_t tgt = cc;
+        if (__longlong_as_double((long long)best[j]) > 0.0) {
+            tgt = bestc[j];
+            if (a.csize[cc] == 1 && tgt > cc && a.csize[tgt] == 1) tgt = cc;
+        }
+        a.target[v] = tgt;
+        a.cw[v] = ecc;
+        if (a.frozen) a.frozen[v] = ecc;
+    }
 }
 
 __global__ void k_hbm_table_init(int32_t* keys, double* vals, int64_t n)
@@ -2090,6 +2251,7 @@ struct Phase {
     template <class K>
     void hub_sweep(const MoveArgs& a, const ColorSweep& cs, int64_t total, int cbits, int kbits)
     {
+        static_assert(kSplitEdges, "the hub keys read the 32-bit tail and community copies");
         const int32_t nh = (int32_t)cs.hub_n;
         const int32_t* hubs = (const int32_t*)(lists.p + cs.hub_off);
         const int64_t* hoffs = hub_offs[cs.index].p;
@@ -2101,24 +2263,46 @@ struct Phase {
         int32_t* head = (int32_t*)scratch(ctx, "hs_head", 4 * t1);
         int32_t* pos = (int32_t*)scratch(ctx, "hs_pos", 4 * t1);
         int32_t* starts = (int32_t*)scratch(ctx, "hs_starts", 4 * t1 + 4);
-        VT_LAUNCH(ctx, sizeof(K) == 4 ? "hub_keys32" : "hub_keys64", k_hub_keys<K>, (unsigned)std::min<int64_t>(nh, 16384),
-                  256, 0, hubs, nh, hoffs,
-                  (const int64_t*)g->offs, (const DEdge*)g->edges, (const int64_t*)curr.p, cbits, k0, v0);
-        bool alt = false;
-        if constexpr (sizeof(K) == 4) {
-            alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, hoffs + nh, 1, total, kbits);
-        } else {
-            sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, kbits, (uint64_t*)k1, v1, &alt);
+        double* own = (double*)scratch(ctx, "hs_own", 16 * (size_t)nh);
+        unsigned long long* best = (unsigned long long*)scratch(ctx, "hs_best", 8 * (size_t)nh);
+        int32_t* bestc = (int32_t*)scratch(ctx, "hs_bestc", 4 * (size_t)nh);
+        double* gains = (double*)scratch(ctx, "hs_gains", 8 * t1);
+        const K* ks;
+        const uint32_t* vs;
+        {
+            TimedScope ts(ctx, "local_move_hub_sort");
+            VT_LAUNCH(ctx, sizeof(K) == 4 ? "hub_keys32" : "hub_keys64", k_hub_keys<K>, grid_for(total, 256, 16384), 256, 0,
+                      hubs, nh, hoffs, (const int64_t*)g->offs, (const int32_t*)tail32.p, (const int32_t*)curr32.p, cbits, k0,
+                      v0);
+            bool alt = false;
+            if constexpr (sizeof(K) == 4) {
+                alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, hoffs + nh, 1, total, kbits);
+            } else {
+                sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, kbits, (uint64_t*)k1, v1, &alt);
+            }
+            ks = alt ? k1 : k0;
+            vs = alt ? v1 : v0;
+            VT_LAUNCH(ctx, "hub_heads", k_hub_heads<K>, grid_for(total, 256, 16384), 256, 0, ks, total, head);
+            if constexpr (sizeof(K) == 4) dev_scan_i32(ctx, head, pos, hoffs + nh, total, nullptr, "hs_part");
+            else exclusive_scan_i32(ctx, head, pos, total);
+            VT_LAUNCH(ctx, "hub_starts", k_hub_starts, grid_for(total, 256, 16384), 256, 0, (const int32_t*)head,
+                      (const int32_t*)pos, total, starts);
         }
-        const K* ks = alt ? k1 : k0;
-        const uint32_t* vs = alt ? v1 : v0;
-        VT_LAUNCH(ctx, "hub_heads", k_hub_heads<K>, grid_for(total, 256, 16384), 256, 0, ks, total, head);
-        if constexpr (sizeof(K) == 4) dev_scan_i32(ctx, head, pos, hoffs + nh, total, nullptr, "hs_part");
-        else exclusive_scan_i32(ctx, head, pos, total);
-        VT_LAUNCH(ctx, "hub_starts", k_hub_starts, grid_for(total, 256, 16384), 256, 0, (const int32_t*)head,
-                  (const int32_t*)pos, total, starts);
-        VT_LAUNCH(ctx, "local_move_hub", k_hub_move<K>, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs, nh,
-                  hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)starts, cbits);
+        TimedScope ts(ctx, "local_move_hub");
+        if (nh >= ctx->tuning.hub_block_min) {
+            VT_LAUNCH(ctx, "hub_move_blocks", k_hub_move<K>, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs,
+                      nh, hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)head, (const int32_t*)starts, cbits);
+            return;
+        }
+        VT_LAUNCH(ctx, "hub_own", k_hub_own<K>, grid_for(nh, 64, 4096), 64, 0, a, hubs, nh, hoffs, ks, vs,
+                  (const int32_t*)pos, (const int32_t*)head, (const int32_t*)starts, cbits, own, best, bestc);
+        VT_LAUNCH(ctx, "hub_runs", k_hub_runs<K>, grid_for(total, 256, 16384), 256, 0, a, hubs, nh, hoffs, ks, vs,
+                  (const int32_t*)pos, (const int32_t*)head, (const int32_t*)starts, cbits, (const double*)own, gains, best);
+        VT_LAUNCH(ctx, "hub_pick", k_hub_pick<K>, grid_for(total, 256, 16384), 256, 0, (const int32_t*)pos,
+                  (const int32_t*)head, hoffs, nh, ks, (const int32_t*)starts, cbits, (const double*)gains,
+                  (const unsigned long long*)best, bestc);
+        VT_LAUNCH(ctx, "hub_final", k_hub_final, grid_for(nh, 64, 4096), 64, 0, a, hubs, nh, (const double*)own,
+                  (const unsigned long long*)best, (const int32_t*)bestc);
     }
 
     void sweep(const ColorSweep& cs)
@@ -2590,6 +2774,7 @@ vt_tuning default_tuning()
     t.wide_table_max_degree[0] = 255;  // tables at most half full
     t.wide_table_max_degree[1] = 511;
     t.vertex_records = 0;
+    t.hub_block_min = 1024;
     return t;
 }
 
@@ -2662,6 +2847,8 @@ static void check_tuning(vt_ctx* ctx, const vt_tuning* t)
              w0, w1);
     if (t->vertex_records != 0 && t->vertex_records != 1)
         fail(ctx, VT_ERR_ARG, "vt_tuning: vertex_records = %d, need 0 or 1", t->vertex_records);
+    if (t->hub_block_min < 0 || t->hub_block_min > (1 << 30))
+        fail(ctx, VT_ERR_ARG, "vt_tuning: hub_block_min = %d outside [0, 2^30]", t->hub_block_min);
 }
 
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out)
