@@ -401,15 +401,21 @@ def skewed_form(ctx, args):
     for _ in range(args.prep):
         st.step()
     st.sweep()
-    ctx.set_profiling(True)
-    ctx.reset_stats()
     reps = 5
     t0 = time.perf_counter()
     for _ in range(reps):
         st.sweep()
     wall = (time.perf_counter() - t0) / reps
+    # the per-kernel split from five more sweeps of the same state, timed launch by launch: the
+    # event pair around each of a sweep's ~30 launches adds time of its own, so the wall time
+    # above is taken without them
+    ctx.set_profiling(True)
+    ctx.reset_stats()
+    for _ in range(reps):
+        st.sweep()
     kernels = {}
-    for k in ("local_move", "local_move_rest", "local_move_mid", "local_move_wide", "local_move_hub"):
+    for k in ("local_move", "local_move_rest", "local_move_mid", "local_move_wide", "local_move_hub_sort",
+              "local_move_hub"):
         avg, cnt, _ = ctx.kernel_stats(k)
         if cnt:
             kernels[k] = avg
