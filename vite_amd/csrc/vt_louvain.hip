@@ -1741,12 +1741,16 @@ __global__ void __launch_bounds__(256) k_fold_events_long(const uint32_t* __rest
 }
 
 // currMod = |e_xx*c - a2_x*c*c| (louvain.cpp:2518-2524).
+// movers (or null): the iteration's mover count, copied to movers_out so that one read
+// returns it with the value.
 __global__ void k_modularity(const double* __restrict__ sums, const double* __restrict__ constant,
-                             double* __restrict__ q)
+                             double* __restrict__ q, const int64_t* __restrict__ movers,
+                             int64_t* __restrict__ movers_out)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const double c = constant[0];
         q[0] = fabs((sums[0] * c) - (sums[1] * c * c));
+        if (movers) movers_out[0] = movers[0];
     }
 }
 
@@ -1976,7 +1980,7 @@ struct Phase {
     DBuf<unsigned char> active;
     DBuf<int32_t> lists;     // vertices grouped by (color, degree bucket), vertex order inside
     DBuf<int32_t> corder;    // vertices by color, vertex order inside (event order for color modes)
-    DBuf<double> scal;       // [0..1] sums, [2] constant, [3] q
+    DBuf<double> scal;       // [0..1] sums, [2] constant, [3] q, [4] the iteration's movers (int64 bits)
     DBuf<int64_t> iscal;     // [0] movers of the last apply, [1] of the iteration, [2] scan length
     DBuf<unsigned long long> ucount;
     std::vector<ColorSweep> sweeps;      // per color (one entry in plain/vorder)
@@ -2385,8 +2389,11 @@ struct Phase {
         VT_LAUNCH(ctx, "count_movers", k_add_i64, 1, 64, 0, iscal.p + 1, (const int64_t*)d_movers);
     }
 
+    // The iteration's mover count: read with Q by modularity(), else from the device.
+    int64_t movers_read = -1;
     int64_t iteration_movers()
     {
+        if (movers_read >= 0) return movers_read;
         int64_t m = 0;
         d2h(ctx, &m, iscal.p + 1, 8);
         return m;
@@ -2412,10 +2419,12 @@ struct Phase {
     {
         if (flags & VT_FLAG_SEQUENTIAL_SUMS) seq_sum2(ctx, cw.p, cdeg.p, nv, sums());
         else tree_sum2(ctx, cw.p, cdeg.p, nv, sums());
-        VT_LAUNCH(ctx, "modularity", k_modularity, 1, 64, 0, (const double*)sums(), (const double*)constant(), q());
-        double qh = 0.0;
-        d2h(ctx, &qh, q(), 8);
-        return qh;
+        VT_LAUNCH(ctx, "modularity", k_modularity, 1, 64, 0, (const double*)sums(), (const double*)constant(), q(),
+                  (const int64_t*)(iscal.p + 1), (int64_t*)(scal.p + 4));
+        double qm[2];  // Q and the movers' bits, one read
+        d2h(ctx, qm, q(), 16);
+        std::memcpy(&movers_read, &qm[1], 8);
+        return qm[0];
     }
 };
 
@@ -2488,6 +2497,7 @@ void phase_step(vt_ctx* ctx, vt_phase* st)
     st->numIters++;
     if (ph.et) dzero(ctx, ph.ucount.p, 8);
     dzero(ctx, ph.iscal.p + 1, 8);  // the iteration's movers
+    ph.movers_read = -1;
     if (st->mode == VT_MODE_COLOR) {
         for (int64_t ci = 0; ci < p.num_color; ci++) {
             ph.sweep(ph.sweeps[ci]);
@@ -3128,7 +3138,8 @@ int vt_modularity(vt_ctx* ctx, int64_t nv, const double* cw, const double* cdeg,
         h2d(ctx, s.p + 2, &constant, 8);
         if (flags & VT_FLAG_SEQUENTIAL_SUMS) seq_sum2(ctx, a.p, b.p, nv, s.p);
         else tree_sum2(ctx, a.p, b.p, nv, s.p);
-        VT_LAUNCH(ctx, "modularity", k_modularity, 1, 64, 0, (const double*)s.p, (const double*)(s.p + 2), s.p + 3);
+        VT_LAUNCH(ctx, "modularity", k_modularity, 1, 64, 0, (const double*)s.p, (const double*)(s.p + 2), s.p + 3,
+                  (const int64_t*)nullptr, (int64_t*)nullptr);
         double q = 0;
         d2h(ctx, &q, s.p + 3, 8);
         if (q_out) *q_out = q;
