@@ -180,10 +180,27 @@ void h2d(vt_ctx* ctx, void* dst, const void* src, size_t bytes)
     VT_HIP(ctx, hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream));
     ctx->stage_off += need;
 }
+// Reads of up to kStageMax bytes (the per-iteration scalars, counts) land in the staging ring
+// past its queued copies, as a copy to pinned memory, and are copied out after the wait.
 void d2h(vt_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
-    if (bytes) VT_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (bytes && bytes <= kStageMax) {
+        if (!ctx->stage) {
+            VT_HIP(ctx, hipHostMalloc((void**)&ctx->stage, kStageCap, hipHostMallocDefault));
+            ctx->stage_off = 0;
+        }
+        if (ctx->stage_off + bytes > kStageCap) {
+            VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            ctx->stage_off = 0;
+        }
+        unsigned char* slot = ctx->stage + ctx->stage_off;
+        VT_HIP(ctx, hipMemcpyAsync(slot, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        std::memcpy(dst, slot, bytes);
+    } else {
+        if (bytes) VT_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        VT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     ctx->stage_off = 0;  // the stream has drained: every staged copy has run
 }
 void d2d(vt_ctx* ctx, void* dst, const void* src, size_t bytes)
