@@ -221,6 +221,23 @@ def test_rebuild_from_device_partition(ctx, name):
     assert np.array_equal(r.edges.view(np.uint8), r_o.edges.view(np.uint8))
 
 
+def test_rebuild_rejects_ids_outside_the_graph(ctx):
+    """A caller's partition with a community id of -1 or nv is refused with an error (the
+    partition a phase leaves on the device is not checked: its ids are vertex ids); the
+    context and the graph stay usable."""
+    g, dg = _upload(ctx, "karate")
+    for bad in (-1, g.nv):
+        cv = np.zeros(g.nv, np.int64)
+        cv[5] = bad
+        with pytest.raises(vite_amd.VtError):
+            dg.rebuild(cv)
+    cv = np.arange(g.nv) % 3
+    r = dg.rebuild(cv).download()
+    r_o = po.rebuild(g, cv)
+    assert np.array_equal(r.offs, r_o.offs)
+    assert np.array_equal(r.edges.view(np.uint8), r_o.edges.view(np.uint8))
+
+
 @pytest.mark.parametrize("name", graphs.ALL)
 @pytest.mark.parametrize("nhash,single", [(4, False), (2, False), (4, True), (1, False)])
 def test_color(ctx, name, nhash, single):

@@ -310,7 +310,7 @@ void batch_run(vt_ctx* ctx, vt_batch* b, const vt_run_params* rp, vt_run_result*
                           (const int64_t*)cur->part, (const unsigned long long*)first.p, cur_gid, cnt.p);
                 std::vector<int32_t> hc(m);
                 d2h(ctx, hc.data(), cnt.p, 4 * (size_t)m);
-                cg = rebuild(ctx, cur, cur->part);
+                cg = rebuild(ctx, cur, cur->part, false);
                 cvoff.assign((size_t)m + 1, 0);
                 for (int32_t j = 0; j < m; j++) cvoff[j + 1] = cvoff[j] + hc[j];
                 DBuf<int64_t> di(ctx, m + 1), dout(ctx, m + 1);

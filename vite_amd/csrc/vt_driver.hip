@@ -13,7 +13,6 @@
 namespace vt {
 
 double run_phase(vt_ctx* ctx, vt_graph* g, const vt_phase_params* p, int64_t* cvect_out, int32_t* iters_out);
-vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect);
 int64_t color_graph(vt_ctx* ctx, const vt_graph* g, int64_t nHash, int target_percent, bool single, int64_t* d_colors);
 
 static double now_s()
@@ -93,7 +92,7 @@ void run_loop(vt_ctx* ctx, const vt_graph* g0, const vt_run_params* rp, vt_run_r
 
             if ((currMod - prevMod) > threshold) {
                 if (!rp->one_phase) {
-                    vt_graph* next = rebuild(ctx, dg, dg->part);
+                    vt_graph* next = rebuild(ctx, dg, dg->part, false);
                     if (owned) graph_delete(ctx, owned);
                     owned = next;
                     dg = next;

@@ -243,6 +243,8 @@ bool dev_sort_pairs_u32(vt_ctx* ctx, uint32_t* keys, uint32_t* vals, uint32_t* k
 // ---- graph helpers ---------------------------------------------------------------------
 vt_graph* graph_new(vt_ctx* ctx, int64_t nv, int64_t ne);
 void graph_finalize(vt_ctx* ctx, vt_graph* g);  // computes max_degree
+// The largest offs[i + 1] - offs[i] over i < nv, atomically maxed into *out (cleared first).
+__global__ void k_max_degree(const int64_t* __restrict__ offs, int64_t nv, unsigned long long* __restrict__ out);
 void graph_delete(vt_ctx* ctx, vt_graph* g);
 
 int64_t bits_for(int64_t maxval);
@@ -252,6 +254,8 @@ vt_tuning default_tuning();  // bucket bounds at each kernel's capacity (vt_louv
 void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& seg, const std::vector<double>& lower,
                  const std::vector<double>& thresh, int flags, bool use_graph, std::vector<double>& q_out,
                  std::vector<int32_t>& iters_out, int* rounds_out, int* relists_out = nullptr);
-vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_part);
+// check_ids: d_part comes from a caller and its ids are checked to lie in [0, nv) (one host
+// wait); a partition a phase left on the device needs no check.
+vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_part, bool check_ids);
 
 }  // namespace vt

@@ -584,7 +584,7 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     // every community's coarse row, staged at its offset in the concatenated rows
     DBuf<uint32_t> rkey(ctx, e1);
     DBuf<double> rval(ctx, e1);
-    DBuf<int64_t> rowlen(ctx, c1n), xcnt(ctx, c1n), out_offs(ctx, c1n), xoff(ctx, c1n), tot(ctx, 2);
+    DBuf<int64_t> rowlen(ctx, c1n), xcnt(ctx, c1n), out_offs(ctx, c1n), xoff(ctx, c1n), tot(ctx, 3);
     dzero(ctx, rowlen.p + nc, 8);
     dzero(ctx, xcnt.p + nc, 8);
     if (nc)
@@ -595,13 +595,24 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     exclusive_scan_i64(ctx, xcnt.p, xoff.p, nc + 1);
     VT_LAUNCH(ctx, "rb_totals", k_pair_totals, 1, 64, 0, (const int64_t*)out_offs.p + nc, (const int64_t*)xoff.p + nc,
               tot.p);
-    int64_t totals[2] = {0, 0};
-    d2h(ctx, totals, tot.p, 16);
+    // the coarse graph's largest row is read with its entry count (the overflowing
+    // communities' rows are added below, and the count and the largest row read again)
+    auto max_row = [&]() {
+        dzero(ctx, tot.p + 2, 8);
+        if (nc)
+            VT_LAUNCH(ctx, "max_degree", k_max_degree, grid_for(nc, 256, 512), 256, 0, (const int64_t*)out_offs.p, nc,
+                      (unsigned long long*)(tot.p + 2));
+    };
+    max_row();
+    int64_t totals[3] = {0, 0, 0};
+    d2h(ctx, totals, tot.p, 24);
     if (totals[1]) {
         overflow_rows(ctx, g, newid, nc, bits, cstart.p, mrow.p, moff.p, xcnt.p, xoff.p, totals[1], rkey.p, rval.p,
                       rowlen.p);
         exclusive_scan_i64(ctx, rowlen.p, out_offs.p, nc + 1);
-        d2h(ctx, totals, out_offs.p + nc, 8);
+        d2d(ctx, tot.p, out_offs.p + nc, 8);
+        max_row();
+        d2h(ctx, totals, tot.p, 24);
     }
 
     vt_graph* out = graph_new(ctx, nc, totals[0]);
@@ -611,7 +622,7 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
             VT_LAUNCH(ctx, "rb_row_copy", k_row_copy, grid_for(nc, kCfWaves, 65536), 64 * kCfWaves, 0, nc,
                       (const int64_t*)cstart.p, (const int64_t*)moff.p, (const int64_t*)rowlen.p,
                       (const int64_t*)out_offs.p, (const uint32_t*)rkey.p, (const double*)rval.p, out->edges);
-        graph_finalize(ctx, out);
+        out->max_degree = totals[2];
     } catch (...) {
         graph_delete(ctx, out);
         throw;
@@ -619,7 +630,7 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     return out;
 }
 
-vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect)
+vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect, bool check_ids)
 {
     const int64_t nv = g->nv, ne = g->ne;
     if (ne >= (int64_t(1) << 32)) fail(ctx, VT_ERR_LIMIT, "vt_rebuild: %lld edges; this build supports < 2^32", (long long)ne);
@@ -630,9 +641,11 @@ vt_graph* rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* d_cvect)
     dzero(ctx, bad.p, 16);
     VT_LAUNCH(ctx, "rb_fill", k_fill_u64, grid_for(nv, 256, 8192), 256, 0, first.p, nv, ~0ull);
     VT_LAUNCH(ctx, "rb_first", k_first_vertex, grid_for(nv, 256, 8192), 256, 0, d_cvect, nv, first.p, bad.p);
-    unsigned long long nbad = 0;
-    d2h(ctx, &nbad, bad.p, 8);
-    if (nbad) fail(ctx, VT_ERR_ARG, "vt_rebuild: %llu community ids outside [0, nv)", nbad);
+    if (check_ids) {  // a caller's partition; a phase's own has every id in [0, nv)
+        unsigned long long nbad = 0;
+        d2h(ctx, &nbad, bad.p, 8);
+        if (nbad) fail(ctx, VT_ERR_ARG, "vt_rebuild: %llu community ids outside [0, nv)", nbad);
+    }
     VT_LAUNCH(ctx, "rb_flags", k_first_flags, grid_for(nv, 256, 8192), 256, 0, d_cvect, nv,
               (const unsigned long long*)first.p, flag.p);
     exclusive_scan_i64(ctx, flag.p, rank.p, nv);
@@ -666,7 +679,7 @@ extern "C" int vt_rebuild(vt_ctx* ctx, const vt_graph* g, const int64_t* cvect, 
         } else if (!src) {
             fail(ctx, VT_ERR_STATE, "vt_rebuild: no partition given and none left by vt_louvain_phase");
         }
-        *out = rebuild(ctx, g, src);
+        *out = rebuild(ctx, g, src, cvect != nullptr);
         sync(ctx);
     });
 }
