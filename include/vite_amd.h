@@ -85,10 +85,11 @@ typedef struct {
      * with the neighbour's line.  Pays on graphs whose neighbours are scattered (random
      * reads); 0 (the default): separate arrays. */
     int32_t vertex_records;
-    /* Rows above bucket_max_degree[5] (hubs) are grouped by neighbour community with one
-     * sort per sweep; a sweep with at least this many hubs then gives each hub a workgroup,
-     * one with fewer runs every community run in a flat launch (few large hubs would
-     * leave most of the chip idle).  0 <= hub_block_min <= 2^30. */
+    /* Rows above bucket_max_degree[5] (hubs) are grouped by neighbour community with a sort
+     * per sweep.  A sweep with at least this many hubs gives each hub a workgroup, which
+     * sorts the hub's row (a segmented sort) and moves the hub; one with fewer sorts all hub
+     * entries at once and runs every community run in a flat launch (a few large hubs
+     * would leave most of the chip idle).  0 <= hub_block_min <= 2^30. */
     int32_t hub_block_min;
 } vt_tuning;
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out);

@@ -13,6 +13,7 @@
 // All FP64 here must be compiled with -ffp-contract=off: the reference build forms no
 // FMA, and a contracted gain can flip a near-tie.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <array>
@@ -2257,6 +2258,10 @@ struct Phase {
     {
         static_assert(kSplitEdges, "the hub keys read the 32-bit tail and community copies");
         const int32_t nh = (int32_t)cs.hub_n;
+        // many hubs (the coarse levels): a workgroup per hub sorts its segment and moves it;
+        // few (BA's level 0: a few hundred, the largest with 36 K entries): one sort of all the
+        // entries, and the flat move launches
+        const bool many = nh >= ctx->tuning.hub_block_min;
         const int32_t* hubs = (const int32_t*)(lists.p + cs.hub_off);
         const int64_t* hoffs = hub_offs[cs.index].p;
         const size_t t1 = (size_t)std::max<int64_t>(total, 1);
@@ -2279,7 +2284,19 @@ struct Phase {
                       hubs, nh, hoffs, (const int64_t*)g->offs, (const int32_t*)tail32.p, (const int32_t*)curr32.p, cbits, k0,
                       v0);
             bool alt = false;
-            if constexpr (sizeof(K) == 4) {
+            if (many) {
+                // each hub's entries are already together: sort each hub's segment by the
+                // community bits alone (the hub bits are the same throughout a segment)
+                rocprim::double_buffer<K> kb(k0, k1);
+                rocprim::double_buffer<uint32_t> vb(v0, v1);
+                size_t tb = 0;
+                VT_HIP(ctx, rocprim::segmented_radix_sort_pairs(nullptr, tb, kb, vb, (unsigned)total, (unsigned)nh, hoffs,
+                                                                 hoffs + 1, 0u, (unsigned)cbits, ctx->stream));
+                void* tmp = scratch(ctx, "hs_segsort", tb);
+                VT_HIP(ctx, rocprim::segmented_radix_sort_pairs(tmp, tb, kb, vb, (unsigned)total, (unsigned)nh, hoffs,
+                                                                 hoffs + 1, 0u, (unsigned)cbits, ctx->stream));
+                alt = kb.current() == k1;
+            } else if constexpr (sizeof(K) == 4) {
                 alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, hoffs + nh, 1, total, kbits);
             } else {
                 sort_pairs_u64(ctx, (uint64_t*)k0, v0, total, kbits, (uint64_t*)k1, v1, &alt);
@@ -2293,7 +2310,7 @@ struct Phase {
                       (const int32_t*)pos, total, starts);
         }
         TimedScope ts(ctx, "local_move_hub");
-        if (nh >= ctx->tuning.hub_block_min) {
+        if (many) {
             VT_LAUNCH(ctx, "hub_move_blocks", k_hub_move<K>, (unsigned)std::min<int64_t>(nh, 8192), kHubBlock, 0, a, hubs,
                       nh, hoffs, ks, vs, (const int32_t*)pos, (const int32_t*)head, (const int32_t*)starts, cbits);
             return;
