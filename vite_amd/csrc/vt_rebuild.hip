@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -523,12 +524,23 @@ static void overflow_rows(vt_ctx* ctx, const vt_graph* g, const uint32_t* newid,
     if (nx > 1) {
         rocprim::double_buffer<unsigned long long> kb(k0.p, k1.p);
         rocprim::double_buffer<double> vb(v0.p, v1.p);
+        // Many entries per community (the coarse levels, where nearly every community
+        // overflows): each community's entries, already together at xoff[C1], are sorted by C2
+        // alone, a workgroup each; otherwise one sort of every entry by (C1, C2).  Both are
+        // stable, so the order is the same.
+        const bool seg = nx >= 16 * nc;
+        auto sort = [&](void* tmp, size_t& tb) {
+            if (seg)
+                return rocprim::segmented_radix_sort_pairs(tmp, tb, kb, vb, (unsigned)nx, (unsigned)nc, xoff, xoff + 1, 0u,
+                                                           (unsigned)bits, ctx->stream);
+            return rocprim::radix_sort_pairs(tmp, tb, kb, vb, (size_t)nx, 0, 2 * bits, ctx->stream);
+        };
         size_t tb = 0;
-        VT_HIP(ctx, rocprim::radix_sort_pairs(nullptr, tb, kb, vb, (size_t)nx, 0, 2 * bits, ctx->stream));
+        VT_HIP(ctx, sort(nullptr, tb));
         void* tmp = scratch(ctx, "sort_tmp", tb);
         {
             TimedScope ts(ctx, "rb_sort");
-            VT_HIP(ctx, rocprim::radix_sort_pairs(tmp, tb, kb, vb, (size_t)nx, 0, 2 * bits, ctx->stream));
+            VT_HIP(ctx, sort(tmp, tb));
         }
         ks = kb.current();
         vs = vb.current();
