@@ -71,7 +71,7 @@ int vt_ctx_reset_stats(vt_ctx* ctx);
  * 2 x lanes - 1) or one 64-lane wave with a 2048-slot table (b = 5, at most 1023); longer rows
  * use tables in HBM.  blocks_per_bucket caps the workgroups one bucket gets per launch.
  * Any valid setting gives bit-identical results; only the speed changes.
- * Default {7, 15, 31, 63, 127, 1023}, 16384, {255, 511}, 0, 1024. */
+ * Default {7, 15, 47, 63, 191, 511}, 16384, {255, 511}, 0, 1024. */
 typedef struct {
     int32_t bucket_max_degree[6];
     int32_t blocks_per_bucket;
@@ -87,9 +87,10 @@ typedef struct {
     int32_t vertex_records;
     /* Rows above bucket_max_degree[5] (hubs) are grouped by neighbour community with a sort
      * per sweep.  A sweep with at least this many hubs gives each hub a workgroup, which
-     * sorts the hub's row (a segmented sort) and moves the hub; one with fewer sorts all hub
-     * entries at once and runs every community run in a flat launch (a few large hubs
-     * would leave most of the chip idle).  0 <= hub_block_min <= 2^30. */
+     * sorts the hub's row (a segmented sort) and moves the hub, unless its largest hub holds
+     * more than 1/64 of the hub entries; otherwise it sorts all hub entries at once and runs
+     * every community run in a flat launch (a few large hubs would leave most of the chip
+     * idle).  0 <= hub_block_min <= 2^30; 0 gives every sweep the per-hub form. */
     int32_t hub_block_min;
 } vt_tuning;
 int vt_ctx_get_tuning(const vt_ctx* ctx, vt_tuning* out);

@@ -87,19 +87,18 @@ def test_ba_hubs_color_et_run_exact(ctx, ba_hubs):
 @pytest.mark.timeout(300)
 def test_ba_2p22_hub_key_widths_and_forms(ctx):
     """The hub path's two key widths and two move forms on the skewed-degree graph (BA
-    n = 2^22, m = 8), whose keys are (hub index, neighbour community).  With the default
-    bounds its few hundred rows of degree 1024+ are the hubs and the keys fit 32 bits (22
-    community bits, 9 hub bits), so the device sort runs; with every row above degree 130
-    sent to the hub path, some 17 K hubs need 64-bit keys and the library sort runs.  Each
-    width runs with a workgroup per hub and with the flat launches (hub_block_min 0 and
-    2^30).  All four sweeps, from the state two phase-0 iterations leave, equal the oracle's
-    bit for bit."""
+    n = 2^22, m = 8), whose keys are (hub index, neighbour community).  With rows above
+    degree 1023 as the hubs, its few hundred of them have keys of 32 bits (22 community bits,
+    9 hub bits), so the device sort runs; with every row above degree 130 sent to the hub
+    path, some 17 K hubs need 64-bit keys and the library sort runs.  Each width runs with a
+    workgroup per hub and with the flat launches (hub_block_min 0 and 2^30).  All four
+    sweeps, from the state two phase-0 iterations leave, equal the oracle's bit for bit."""
     dg = ctx.gen_ba(2027, 1 << 22, 8)
     g = _host(dg)
     default = ctx.tuning
     oracle = None
     try:
-        for width, bounds in ((32, default["bucket_max_degree"]), (64, [7, 15, 47, 63, 127, 130])):
+        for width, bounds in ((32, [7, 15, 47, 63, 191, 1023]), (64, [7, 15, 47, 63, 127, 130])):
             for blocks in (False, True):
                 ctx.set_tuning(bucket_max_degree=bounds, hub_block_min=0 if blocks else 1 << 30)
                 ctx.set_profiling(True)

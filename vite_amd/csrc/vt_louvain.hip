@@ -2258,10 +2258,12 @@ struct Phase {
     {
         static_assert(kSplitEdges, "the hub keys read the 32-bit tail and community copies");
         const int32_t nh = (int32_t)cs.hub_n;
-        // many hubs (the coarse levels): a workgroup per hub sorts its segment and moves it;
-        // few (BA's level 0: a few hundred, the largest with 36 K entries): one sort of all the
-        // entries, and the flat move launches
-        const bool many = nh >= ctx->tuning.hub_block_min;
+        // many hubs, none holding much of the work (the coarse levels): a workgroup per hub
+        // sorts its segment and moves it; otherwise (BA's level 0: a few hundred or thousand
+        // hubs, the largest with 36 K of 0.6-1.5 M entries) one sort of all the entries, and
+        // the flat move launches
+        const int32_t hmin = ctx->tuning.hub_block_min;
+        const bool many = hmin == 0 || (nh >= hmin && 64 * g->max_degree <= total);
         const int32_t* hubs = (const int32_t*)(lists.p + cs.hub_off);
         const int64_t* hoffs = hub_offs[cs.index].p;
         const size_t t1 = (size_t)std::max<int64_t>(total, 1);
@@ -2797,6 +2799,10 @@ vt_tuning default_tuning()
 {
     vt_tuning t;
     for (int b = 0; b < 6; b++) t.bucket_max_degree[b] = kBucketCap[b];
+    // rows of degree 512-1023 go to the hub path: on the coarse levels, where thousands of
+    // rows are that long, a workgroup each sorting and moving them beats one wave each with a
+    // 2048-slot table (time to final Q on RGG n = 2^24: 0.1045-0.1049 s against 0.1059-0.1062)
+    t.bucket_max_degree[5] = 511;
     t.blocks_per_bucket = 16384;
     t.wide_table_max_degree[0] = 255;  // tables at most half full
     t.wide_table_max_degree[1] = 511;
