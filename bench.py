@@ -163,25 +163,26 @@ def config1_time_to_q(ctx, args, host):
     return out
 
 
-def iteration_timing(st, V, E, steps=3):
+def iteration_timing(dg, st, V, E, steps=3):
     """Whole Louvain iterations (sweep, the per-community delta folds that replace
     localCupdate / distUpdateLocalCinfo, modularity, rotation) continued from the benchmark
-    state, each timed alone.  Algorithmic bytes: 24E + 56V (sweep) + 8V (mover flags) + 64 B
-    per mover (two 32-byte events) + 24V (modularity)."""
-    import numpy as np
-
+    state, back to back as in a run, each timed alone on the host; each iteration's movers come
+    from the library's per-iteration record afterwards (a download of the state between
+    iterations would evict the Infinity Cache a run keeps warm).  Algorithmic bytes: 24E + 56V
+    (sweep) + 8V (mover flags) + 64 B per mover (two 32-byte events) + 24V (modularity)."""
     its = []
+    st.sweep()  # untimed, changes nothing: the caches as the previous iteration leaves them
     for _ in range(steps):
-        st.sweep()
-        s = st.state()
-        movers = int(np.count_nonzero(s["target"] != s["curr"]))
         t0 = time.perf_counter()
         q, done = st.step()
         sec = time.perf_counter() - t0
-        b = 24.0 * E + 88.0 * V + 64.0 * movers
-        its.append({"ms": 1e3 * sec, "movers": movers, "bytes": b, "hbm_frac": b / sec / 1e9 / HBM_PEAK_GBS, "q": q})
+        its.append({"ms": 1e3 * sec, "q": q})
         if done:
             break
+    _, moves = dg.last_trace()
+    for it, movers in zip(its, moves[-len(its):]):
+        b = 24.0 * E + 88.0 * V + 64.0 * int(movers)
+        it.update(movers=int(movers), bytes=b, hbm_frac=b / (it["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
     ms = sum(i["ms"] for i in its) / len(its)
     return {"ms_per_iteration": ms, "edges_per_s": E / (ms * 1e-3),
             "hbm_frac": sum(i["hbm_frac"] for i in its) / len(its), "iterations": its,
@@ -688,7 +689,7 @@ def main():
     state = st.state()
     per_iter = None
     if not args.no_iter:
-        per_iter = iteration_timing(st, V, E)
+        per_iter = iteration_timing(dg, st, V, E)
         log(f"whole iterations: {per_iter['ms_per_iteration']:.3f} ms each "
             f"({100 * per_iter['hbm_frac']:.1f}% of HBM peak in algorithmic bytes)")
     st.end(want_cvect=False)
