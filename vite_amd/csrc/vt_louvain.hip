@@ -1216,11 +1216,11 @@ __device__ __forceinline__ double hub_run_fold(const DEdge* __restrict__ edges, 
     return acc;
 }
 
-// The hub local move as four flat launches, for sweeps with few hubs (a workgroup per hub
-// would leave most of the chip idle: a few hundred hubs, the largest with tens of thousands
-// of runs).  Runs are the sorted hub
-// entries' runs of equal key, starts[] their bounds, pos[] the run index of each entry;
-// own[2j], own[2j+1] = hub j's own-community counter and self-loop weight.
+// The hub local move as four flat launches, for sweeps with few hubs or one hub holding much
+// of the work (a workgroup per hub would leave most of the chip idle: a few hundred hubs, the
+// largest with tens of thousands of runs).  Runs are the sorted hub entries' runs of equal
+// key, starts[] their bounds, pos[] the run index of each entry; own[2j], own[2j+1] = hub j's
+// own-community counter and self-loop weight.
 __device__ __forceinline__ int32_t hub_run_count(const int64_t* __restrict__ hoffs, int32_t nh,
                                                  const int32_t* __restrict__ pos, const int32_t* __restrict__ head)
 {
