@@ -71,7 +71,7 @@ int vt_ctx_reset_stats(vt_ctx* ctx);
  * 2 x lanes - 1) or one 64-lane wave with a 2048-slot table (b = 5, at most 1023); longer rows
  * use tables in HBM.  blocks_per_bucket caps the workgroups one bucket gets per launch.
  * Any valid setting gives bit-identical results; only the speed changes.
- * Default {7, 15, 47, 63, 191, 511}, 16384, {255, 511}, 0, 1024. */
+ * Default {7, 15, 47, 63, 191, 511}, 65536, {255, 511}, 0, 1024. */
 typedef struct {
     int32_t bucket_max_degree[6];
     int32_t blocks_per_bucket;

@@ -2803,7 +2803,9 @@ vt_tuning default_tuning()
     // rows are that long, a workgroup each sorting and moving them beats one wave each with a
     // 2048-slot table (time to final Q on RGG n = 2^24: 0.1045-0.1049 s against 0.1059-0.1062)
     t.bucket_max_degree[5] = 511;
-    t.blocks_per_bucket = 16384;
+    // the largest cap measured best over whole threshold-cycling runs (time to final Q on RGG
+    // n = 2^24: 0.1035-0.1038 s against 0.1045-0.1047 s at 16384 and 0.106 s at 2048)
+    t.blocks_per_bucket = 65536;
     t.wide_table_max_degree[0] = 255;  // tables at most half full
     t.wide_table_max_degree[1] = 511;
     t.vertex_records = 0;
