@@ -1166,6 +1166,24 @@ __global__ void k_hub_keys(const int32_t* __restrict__ hubs, int32_t nh, const i
     }
 }
 
+// The same keys with a workgroup per hub row (sweeps with many hubs: no search, and each
+// row's entries are read in order).
+template <class K>
+__global__ void k_hub_keys_rows(const int32_t* __restrict__ hubs, int32_t nh, const int64_t* __restrict__ hoffs,
+                                const int64_t* __restrict__ offs, const int32_t* __restrict__ tail32,
+                                const int32_t* __restrict__ curr32, int cbits, K* __restrict__ keys,
+                                uint32_t* __restrict__ vals)
+{
+    for (int32_t j = blockIdx.x; j < nh; j += gridDim.x) {
+        const int64_t q0 = hoffs[j], n = hoffs[j + 1] - q0, e0 = offs[hubs[j]];
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int64_t k = e0 + i;
+            keys[q0 + i] = ((K)j << cbits) | (K)(uint32_t)curr32[tail32[k]];
+            vals[q0 + i] = (uint32_t)k;
+        }
+    }
+}
+
 template <class K>
 __global__ void k_hub_heads(const K* __restrict__ keys, int64_t n, int32_t* __restrict__ head)
 {
@@ -2282,9 +2300,13 @@ struct Phase {
         const uint32_t* vs;
         {
             TimedScope ts(ctx, "local_move_hub_sort");
-            VT_LAUNCH(ctx, sizeof(K) == 4 ? "hub_keys32" : "hub_keys64", k_hub_keys<K>, grid_for(total, 256, 16384), 256, 0,
-                      hubs, nh, hoffs, (const int64_t*)g->offs, (const int32_t*)tail32.p, (const int32_t*)curr32.p, cbits, k0,
-                      v0);
+            const char* kname = sizeof(K) == 4 ? "hub_keys32" : "hub_keys64";
+            if (many)
+                VT_LAUNCH(ctx, kname, k_hub_keys_rows<K>, (unsigned)std::min<int64_t>(nh, 16384), 256, 0, hubs, nh, hoffs,
+                          (const int64_t*)g->offs, (const int32_t*)tail32.p, (const int32_t*)curr32.p, cbits, k0, v0);
+            else
+                VT_LAUNCH(ctx, kname, k_hub_keys<K>, grid_for(total, 256, 16384), 256, 0, hubs, nh, hoffs,
+                          (const int64_t*)g->offs, (const int32_t*)tail32.p, (const int32_t*)curr32.p, cbits, k0, v0);
             bool alt = false;
             if (many) {
                 // each hub's entries are already together: sort each hub's segment by the
