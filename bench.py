@@ -1,6 +1,6 @@
 """Benchmark: Louvain local-move sweep on one MI355X.
 
-Workload (the north-star graph): seeded random geometric graph, n = 2^24 vertices in
+Workload: seeded random geometric graph, n = 2^24 vertices in
 [0,1]^2, radius sqrt(2 ln n / (pi n)), plus 1% extra uniform-random edges with U(0,1)
 weights; FP64 weights, 64-bit ids; generated on the device.  Phase 0 runs with the first
 threshold of threshold cycling (1e-3).
