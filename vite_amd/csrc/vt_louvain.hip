@@ -2186,7 +2186,8 @@ struct Phase {
         for (int64_t c = 0; c < ncol; c++) sweeps[c].index = (int32_t)c;
         // Hub rows: by sorting, unless the sweep must stay free of library sorts (a batch
         // replays its sweeps from captured graphs) or edge indexes exceed 32 bits.
-        hub_sort = hub_sort && !gid && g->ne < (int64_t(1) << 32);
+        // the sorted path keeps run positions in 32 bits: hub entries (at most ne) below 2^31
+        hub_sort = hub_sort && !gid && g->ne < (int64_t(1) << 31);
         hub_offs.clear();
         hub_total.assign(ncol, 0);
         if (hub_sort) {
