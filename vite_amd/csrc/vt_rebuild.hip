@@ -92,26 +92,13 @@ __global__ void k_run_starts(const int64_t* __restrict__ head, const int64_t* __
 // kCfMaxKeys distinct keys takes a stable radix sort of its entries by (C1, C2) instead,
 // whose runs are folded left to right.
 
-// Members per community: each wave takes 64 consecutive vertices, and the first lane of each
-// run of equal ids adds the run's length (one atomic per run instead of per vertex).
-__global__ void k_count_members(const uint32_t* __restrict__ newid, int64_t nv, int64_t* __restrict__ cnt)
+// Each community's first position among the members sorted by community: every community
+// id below nc has a member, so cstart[c] is where the run of c begins, and cstart[nc] = nv.
+__global__ void k_member_starts(const uint32_t* __restrict__ keys, int64_t nv, int64_t nc, int64_t* __restrict__ cstart)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < nv; base += stride) {
-        const int64_t v = base + lane;
-        const bool valid = v < nv;  // the valid lanes are a prefix of the wave
-        const uint32_t c = valid ? newid[v] : 0xFFFFFFFFu;
-        const uint32_t before = __shfl_up(c, 1);
-        const bool head = valid && (lane == 0 || before != c);
-        const unsigned long long heads = __ballot(head);
-        const int nvalid = __popcll(__ballot(valid));
-        if (head) {
-            const unsigned long long later = lane == 63 ? 0ull : heads >> (lane + 1);
-            const int end = later ? lane + 1 + __builtin_ctzll(later) : nvalid;
-            atomicAdd((unsigned long long*)&cnt[c], (unsigned long long)(end - lane));
-        }
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cstart[nc] = nv;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nv; p += (int64_t)gridDim.x * blockDim.x)
+        if (p == 0 || keys[p] != keys[p - 1]) cstart[keys[p]] = p;
 }
 
 __global__ void k_iota_u32(uint32_t* __restrict__ p, int64_t n)
@@ -578,15 +565,13 @@ static vt_graph* rebuild_fold(vt_ctx* ctx, const vt_graph* g, const uint32_t* ne
     // member's start in its community's concatenated row
     DBuf<int64_t> cstart(ctx, c1n), moff(ctx, n1 + 1), mrow(ctx, n1);
     {
-        DBuf<int64_t> ccnt(ctx, c1n);
-        dzero(ctx, ccnt.p, 8 * c1n);
-        VT_LAUNCH(ctx, "rb_members", k_count_members, grid_for(nv, 256, 8192), 256, 0, newid, nv, ccnt.p);
-        exclusive_scan_i64(ctx, ccnt.p, cstart.p, nc + 1);
         DBuf<uint32_t> mk(ctx, n1), mv(ctx, n1), mk2(ctx, n1), mv2(ctx, n1);
         d2d(ctx, mk.p, newid, 4 * (size_t)nv);
         VT_LAUNCH(ctx, "rb_iota", k_iota_u32, grid_for(nv, 256, 8192), 256, 0, mv.p, nv);
         bool malt = false;
         sort_pairs_u32(ctx, mk.p, mv.p, nv, bits, mk2.p, mv2.p, &malt);
+        VT_LAUNCH(ctx, "rb_member_starts", k_member_starts, grid_for(nv, 256, 8192), 256, 0,
+                  (const uint32_t*)(malt ? mk2.p : mk.p), nv, nc, cstart.p);
         DBuf<int64_t> mdeg(ctx, n1 + 1);
         VT_LAUNCH(ctx, "rb_mdeg", k_member_degrees, grid_for(nv + 1, 256, 8192), 256, 0,
                   (const uint32_t*)(malt ? mv2.p : mv.p), nv, (const int64_t*)g->offs, mdeg.p, mrow.p);
