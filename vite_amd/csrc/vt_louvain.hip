@@ -1630,8 +1630,10 @@ __global__ void k_scan_total(const int32_t* __restrict__ flags, const int32_t* _
 
 __global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, const int32_t* __restrict__ flags,
                               const int32_t* __restrict__ pos, const int64_t* __restrict__ target,
-                              const int64_t* __restrict__ curr, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+                              const int64_t* __restrict__ curr, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                              unsigned long long* __restrict__ long_runs)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *long_runs = 0;  // k_fold_events marks long runs
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         if (!flags[p]) continue;
         const int64_t v = order ? order[p] : p;
@@ -1643,25 +1645,15 @@ __global__ void k_emit_events(const int32_t* __restrict__ order, int64_t n, cons
     }
 }
 
-// Each sorted event's contribution, for all events at once: +k_v when v arrives in the
-// event's community, -k_v when it leaves (a - x == a + (-x) exactly), and +-1 to the member
-// count.  The per-community folds below then read them as a stream.
-__global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n,
-                               const int64_t* __restrict__ d_movers, const int64_t* __restrict__ target,
-                               const double* __restrict__ vdeg, uint32_t none, double* __restrict__ delta,
-                               int8_t* __restrict__ dir, unsigned long long* __restrict__ long_runs)
+// One sorted event's contribution: +k_v when v arrives in the event's community c (its target
+// is c), -k_v when it leaves (a - x == a + (-x) exactly); s = +-1, the member-count change.
+__device__ __forceinline__ double event_delta(uint32_t c, uint32_t v, const int64_t* __restrict__ target,
+                                              const double* __restrict__ vdeg, int& s)
 {
-    if (d_movers) n = 2 * d_movers[0];  // the count the device computed (a replayed graph's n is fixed)
-    if (blockIdx.x == 0 && threadIdx.x == 0) *long_runs = 0;  // k_fold_events marks long runs
-    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t c = keys[j];
-        if (c == none) continue;
-        const uint32_t v = vals[j];
-        const bool in = (uint64_t)target[v] == (uint64_t)c;
-        const double k = vdeg[v];
-        delta[j] = in ? k : -k;
-        dir[j] = in ? 1 : -1;
-    }
+    const bool in = (uint64_t)target[v] == (uint64_t)c;
+    const double k = vdeg[v];
+    s = in ? 1 : -1;
+    return in ? k : -k;
 }
 
 // Events sorted stably by community keep processing order inside a community, so the
@@ -1673,10 +1665,10 @@ __global__ void k_event_deltas(const uint32_t* __restrict__ keys, const uint32_t
 // run held a whole launch for 7 ms on BA n = 2^22.
 constexpr int64_t kFoldLongRun = 1024;
 
-__global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* __restrict__ delta,
-                              const int8_t* __restrict__ dir, int64_t n, const int64_t* __restrict__ d_movers,
-                              double* __restrict__ cdeg, int64_t* __restrict__ csize, uint32_t none,
-                              unsigned long long* __restrict__ long_runs)
+__global__ void k_fold_events(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                              const int64_t* __restrict__ target, const double* __restrict__ vdeg, int64_t n,
+                              const int64_t* __restrict__ d_movers, double* __restrict__ cdeg,
+                              int64_t* __restrict__ csize, uint32_t none, unsigned long long* __restrict__ long_runs)
 {
     if (d_movers) n = 2 * d_movers[0];
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
@@ -1695,16 +1687,21 @@ __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* _
         int64_t k = j;
         // keys ascend, so keys[k + 3] == c means keys[k..k+3] all are c
         for (; k + 4 <= n && keys[k + 3] == c; k += 4) {
-            const double d0 = delta[k], d1 = delta[k + 1], d2 = delta[k + 2], d3 = delta[k + 3];
+            int s0, s1, s2, s3;
+            const double d0 = event_delta(c, vals[k], target, vdeg, s0);
+            const double d1 = event_delta(c, vals[k + 1], target, vdeg, s1);
+            const double d2 = event_delta(c, vals[k + 2], target, vdeg, s2);
+            const double d3 = event_delta(c, vals[k + 3], target, vdeg, s3);
             acc = acc + d0;
             acc = acc + d1;
             acc = acc + d2;
             acc = acc + d3;
-            cnt += dir[k] + dir[k + 1] + dir[k + 2] + dir[k + 3];
+            cnt += s0 + s1 + s2 + s3;
         }
         for (; k < n && keys[k] == c; k++) {
-            acc = acc + delta[k];
-            cnt += dir[k];
+            int s0;
+            acc = acc + event_delta(c, vals[k], target, vdeg, s0);
+            cnt += s0;
         }
         cdeg[c] = deg0 + acc;
         csize[c] = size0 + cnt;
@@ -1717,8 +1714,9 @@ __global__ void k_fold_events(const uint32_t* __restrict__ keys, const double* _
 // which leaves the sum unchanged) -- the same left fold from 0.0 as k_fold_events.  The
 // member-count changes are integers, summed in any order.
 __global__ void __launch_bounds__(256) k_fold_events_long(const uint32_t* __restrict__ keys,
-                                                          const double* __restrict__ delta,
-                                                          const int8_t* __restrict__ dir, int64_t n,
+                                                          const uint32_t* __restrict__ vals,
+                                                          const int64_t* __restrict__ target,
+                                                          const double* __restrict__ vdeg, int64_t n,
                                                           const int64_t* __restrict__ d_movers,
                                                           double* __restrict__ cdeg, int64_t* __restrict__ csize,
                                                           uint32_t none, const unsigned long long* __restrict__ long_runs)
@@ -1743,8 +1741,9 @@ __global__ void __launch_bounds__(256) k_fold_events_long(const uint32_t* __rest
             for (int64_t k = j0;; k += 64) {
                 const int64_t idx = k + lane;
                 const bool in = idx < n && keys[idx] == c;
-                const double d = in ? delta[idx] : -0.0;
-                int v = in ? (int)dir[idx] : 0;
+                double d = -0.0;
+                int v = 0;
+                if (in) d = event_delta(c, vals[idx], target, vdeg, v);
 #pragma unroll
                 for (int u = 0; u < 64; u++) acc = acc + readlane_f64(d, u);
                 for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -2424,10 +2423,9 @@ struct Phase {
         uint32_t* k1 = (uint32_t*)scratch(ctx, "ev_k1", 4 * (size_t)ne);
         uint32_t* v1 = (uint32_t*)scratch(ctx, "ev_v1", 4 * (size_t)ne);
         VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(n, 256, 8192), 256, 0, order, n, (const int32_t*)flags,
-                  (const int32_t*)pos, target.p, curr.p, k0, v0);
+                  (const int32_t*)pos, target.p, curr.p, k0, v0, ucount.p + 1);
         const bool alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, d_movers, 2, ne, (int)bits_for(nv > 1 ? nv - 1 : 1));
-        fold_events(alt ? k1 : k0, alt ? v1 : v0, ne, 0xFFFFFFFFu, (double*)scratch(ctx, "ev_d", 8 * (size_t)ne),
-                    (int8_t*)scratch(ctx, "ev_s", (size_t)ne), dst_cdeg, dst_csize, d_movers);
+        fold_events(alt ? k1 : k0, alt ? v1 : v0, ne, 0xFFFFFFFFu, dst_cdeg, dst_csize, d_movers);
         VT_LAUNCH(ctx, "count_movers", k_add_i64, 1, 64, 0, iscal.p + 1, (const int64_t*)d_movers);
     }
 
@@ -2444,16 +2442,15 @@ struct Phase {
     // localCinfo += the sorted events' folds, per community in processing order.
     // ne: the event count, or its upper bound when d_movers (the mover count on the device)
     // is given.
-    void fold_events(const uint32_t* ks, const uint32_t* vs, int64_t ne, uint32_t none, double* delta, int8_t* dir,
-                     double* dst_cdeg, int64_t* dst_csize, const int64_t* d_movers = nullptr)
+    // The long-run flag (ucount[1]) was cleared by the emission of these events.
+    void fold_events(const uint32_t* ks, const uint32_t* vs, int64_t ne, uint32_t none, double* dst_cdeg,
+                     int64_t* dst_csize, const int64_t* d_movers = nullptr)
     {
-        VT_LAUNCH(ctx, "event_deltas", k_event_deltas, grid_for(ne, 256, 16384), 256, 0, ks, vs, ne, d_movers,
-                  (const int64_t*)target.p, (const double*)vdeg.p, none, delta, dir, ucount.p + 1);
-        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, (const double*)delta,
-                  (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none, ucount.p + 1);
+        VT_LAUNCH(ctx, "fold_events", k_fold_events, grid_for(ne, 256, 16384), 256, 0, ks, vs, (const int64_t*)target.p,
+                  (const double*)vdeg.p, ne, d_movers, dst_cdeg, dst_csize, none, ucount.p + 1);
         if (ne >= kFoldLongRun)
-            VT_LAUNCH(ctx, "fold_events_long", k_fold_events_long, grid_for(ne, 256, 4096), 256, 0, ks,
-                      (const double*)delta, (const int8_t*)dir, ne, d_movers, dst_cdeg, dst_csize, none,
+            VT_LAUNCH(ctx, "fold_events_long", k_fold_events_long, grid_for(ne, 256, 4096), 256, 0, ks, vs,
+                      (const int64_t*)target.p, (const double*)vdeg.p, ne, d_movers, dst_cdeg, dst_csize, none,
                       (const unsigned long long*)(ucount.p + 1));
     }
 
@@ -2669,8 +2666,6 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
     uint32_t* v0 = (uint32_t*)scratch(ctx, "bev_v0", 4 * (size_t)nev);
     uint32_t* k1 = (uint32_t*)scratch(ctx, "bev_k1", 4 * (size_t)nev);
     uint32_t* v1 = (uint32_t*)scratch(ctx, "bev_v1", 4 * (size_t)nev);
-    double* dl = (double*)scratch(ctx, "bev_d", 8 * (size_t)nev);
-    int8_t* dr = (int8_t*)scratch(ctx, "bev_s", (size_t)nev);
     int32_t* mflags = (int32_t*)scratch(ctx, "bmv_flags", 4 * (size_t)std::max<int64_t>(nv, 1));
     int32_t* mpos = (int32_t*)scratch(ctx, "bmv_pos", 4 * (size_t)std::max<int64_t>(nv, 1));
     const uint32_t none = (uint32_t)nv;
@@ -2714,12 +2709,12 @@ void batch_phase(vt_ctx* ctx, vt_graph* u, const int32_t* d_gid, const SegSums& 
             dev_scan_i32(ctx, mflags, mpos, d_nlive, nlive, d_movers, "bmv_part");
             VT_LAUNCH(ctx, "emit_events", k_emit_events, grid_for(nlive, 256, 8192), 256, 0, (const int32_t*)live.p,
                       nlive, (const int32_t*)mflags, (const int32_t*)mpos, (const int64_t*)ph.target.p,
-                      (const int64_t*)ph.curr.p, k0, v0);
+                      (const int64_t*)ph.curr.p, k0, v0, ph.ucount.p + 1);
         } else {
             dzero(ctx, d_movers, 8);  // every graph has left: no events
         }
         const bool alt = dev_sort_pairs_u32(ctx, k0, v0, k1, v1, d_movers, 2, nev, bits);
-        ph.fold_events(alt ? k1 : k0, alt ? v1 : v0, 2 * nv, none, dl, dr, ph.cdeg.p, ph.csize.p, d_movers);
+        ph.fold_events(alt ? k1 : k0, alt ? v1 : v0, 2 * nv, none, ph.cdeg.p, ph.csize.p, d_movers);
         if (seq) seg_seq_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p, (const unsigned char*)gact.p);
         else seg_tree_sum2(ctx, seg, ph.cw.p, ph.cdeg.p, gsums.p, (const unsigned char*)gact.p);
         dzero(ctx, status.p, 16);
